@@ -1,0 +1,8 @@
+"""MI355X-native HODLR engine: drop-in for the HODLR_Matrix path of
+amiraa127/Dense_HODLR (kernel/dense constructors, partial-pivot ACA or SVD
+compression, recSM/recLU solves, log-determinant), built as hand-written HIP
+kernels for gfx950 behind the C ABI in include/dense_hodlr.h."""
+from ._lib import HodlrError, lib
+from .hodlr_matrix import HODLR_Matrix
+
+__all__ = ["HODLR_Matrix", "HodlrError", "lib"]

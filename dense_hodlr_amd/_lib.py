@@ -1,0 +1,130 @@
+"""ctypes binding of libdense_hodlr_amd.so (declarations in include/dense_hodlr.h).
+
+The library is loaded from dense_hodlr_amd/lib/ only.  There is no fallback:
+if it is missing, or no GPU is usable, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdense_hodlr_amd.so")
+
+HODLR_OK = 0
+ERRORS = {
+    -1: "HODLR_ERR_ARG",
+    -2: "HODLR_ERR_STATE",
+    -3: "HODLR_ERR_DEVICE",
+    -4: "HODLR_ERR_NOMEM",
+    -5: "HODLR_ERR_NUMERIC",
+    -6: "HODLR_ERR_UNSUPPORTED",
+}
+
+KERNELS = {"gaussian": 0, "matern32": 1, "se": 2, "imq": 3, "quadratic": 4, "exponential": 5}
+LR_METHODS = {"partialPiv_ACA": 0, "SVD": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1}
+FACTOR_SM, FACTOR_LU = 0, 1
+
+# every symbol include/dense_hodlr.h declares
+EXPORTS = [
+    "hodlr_create_kernel", "hodlr_create_kernel_callback", "hodlr_create_dense", "hodlr_destroy",
+    "hodlr_set_lr_method", "hodlr_set_tolerance", "hodlr_set_min_pivot", "hodlr_set_precision",
+    "hodlr_set_refinement", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
+    "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
+    "hodlr_matvec", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
+    "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
+]
+
+
+class KernelDesc(C.Structure):
+    _fields_ = [
+        ("family", C.c_int),
+        ("dim", C.c_int),
+        ("points", C.POINTER(C.c_double)),
+        ("length_scale", C.c_double),
+        ("diag", C.c_double),
+        ("diag_mode", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("num_levels", C.c_int),
+        ("num_nodes", C.c_int),
+        ("num_leaves", C.c_int),
+        ("aca_iterations", C.c_int),
+        ("compress_time", C.c_double),
+        ("factor_time", C.c_double),
+        ("solve_time", C.c_double),
+        ("refine_time", C.c_double),
+        ("refine_iterations", C.c_int),
+        ("last_residual", C.c_double),
+        ("max_rank", C.c_int * 64),
+        ("mean_rank", C.c_double * 64),
+        ("device_bytes", C.c_longlong),
+    ]
+
+
+CALLBACK = C.CFUNCTYPE(C.c_double, C.c_int, C.c_int, C.c_void_p)
+
+_lib = None
+
+
+class HodlrError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+def lib():
+    """Load the library (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing; build it with `python -m dense_hodlr_amd.build`")
+    L = C.CDLL(LIB_PATH)
+    dp = C.POINTER(C.c_double)
+    ip = C.POINTER(C.c_int)
+    h = C.c_void_p
+    hp = C.POINTER(C.c_void_p)
+    sig = {
+        "hodlr_create_kernel": (C.c_int, [hp, C.c_int, C.POINTER(KernelDesc), C.c_int]),
+        "hodlr_create_kernel_callback": (C.c_int, [hp, C.c_int, CALLBACK, C.c_void_p, C.c_int]),
+        "hodlr_create_dense": (C.c_int, [hp, C.c_int, dp, C.c_int, C.c_int, C.c_int]),
+        "hodlr_destroy": (C.c_int, [h]),
+        "hodlr_set_lr_method": (C.c_int, [h, C.c_int]),
+        "hodlr_set_tolerance": (C.c_int, [h, C.c_double]),
+        "hodlr_set_min_pivot": (C.c_int, [h, C.c_double]),
+        "hodlr_set_precision": (C.c_int, [h, C.c_int]),
+        "hodlr_set_refinement": (C.c_int, [h, C.c_double, C.c_int]),
+        "hodlr_compress": (C.c_int, [h]),
+        "hodlr_factorize": (C.c_int, [h, C.c_int]),
+        "hodlr_solve_sm": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_solve_lu": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_solve_device": (C.c_int, [h, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_void_p, C.c_int]),
+        "hodlr_logabsdet": (C.c_int, [h, dp]),
+        "hodlr_determinant": (C.c_int, [h, dp]),
+        "hodlr_matvec": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_get_stats": (C.c_int, [h, C.POINTER(Stats)]),
+        "hodlr_rows": (C.c_int, [h]),
+        "hodlr_num_nodes": (C.c_int, [h]),
+        "hodlr_get_nodes": (C.c_int, [h, ip, ip, ip, ip, ip, ip]),
+        "hodlr_get_factor": (C.c_int, [h, C.c_int, C.c_int, dp]),
+        "hodlr_last_error": (C.c_char_p, [h]),
+        "hodlr_version": (C.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc, handle=None):
+    if rc < 0:
+        msg = lib().hodlr_last_error(handle)
+        raise HodlrError(rc, msg.decode() if msg else "")
+    return rc

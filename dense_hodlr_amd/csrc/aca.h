@@ -1,0 +1,52 @@
+/*
+ * Level-batched partial-pivot ACA (see aca.hip).
+ */
+#ifndef DENSE_HODLR_ACA_H
+#define DENSE_HODLR_ACA_H
+
+#include "common.h"
+
+namespace hodlr {
+
+constexpr int ACA_MAX_LEVELS = 32;
+constexpr int MAX_FIN_CHUNKS = 64;  // Gram columns up to 64 * 256 = 16384
+
+/* Device panels of the levels being compressed.  W[l], V[l]: ld x cap[l]
+ * column-major; G[l]: per block two cap x cap Gram matrices; P[l]: partial
+ * slots of (cap + 4) doubles. */
+struct AcaLevels {
+  double* W[ACA_MAX_LEVELS];
+  double* V[ACA_MAX_LEVELS];
+  double* G[ACA_MAX_LEVELS];
+  double* P[ACA_MAX_LEVELS];
+  int cap[ACA_MAX_LEVELS];
+  int ld;
+};
+
+struct AcaLaunch {
+  const AcaBlock* blocks;
+  AcaState* state;
+  uint32_t* bits;
+  const int* wg2blk_row;
+  const int* wg2blk_col;
+  const int* wg2blk_fin;
+  const int* wg2chunk_fin;
+  const int* nchunk;
+  double* fin_part;
+  int* counters;
+  int n_row_wg, n_col_wg, n_fin_wg;
+  int jpt, ipt;
+  double tol;
+  int min_rank, max_rank, min_pivot;
+};
+
+void aca_launch_init(const AcaBlock* d_blocks, AcaState* d_state, uint32_t* d_bits, int nblocks,
+                     hipStream_t s);
+void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
+                          hipStream_t s);
+void aca_launch_post(const AcaLevels& L, const AcaBlock* d_blocks, AcaState* d_state,
+                     const EntrySource* d_srcs, int* d_ranks, int nblocks, hipStream_t s);
+
+}  // namespace hodlr
+
+#endif

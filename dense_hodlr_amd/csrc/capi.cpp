@@ -1,0 +1,297 @@
+/*
+ * C ABI of the HODLR engine (declarations and reference citations in
+ * include/dense_hodlr.h).  Exceptions never cross the boundary: every entry
+ * point maps them to a status code and keeps the message.
+ */
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+#include <string>
+
+#include "../../include/dense_hodlr.h"
+#include "engine.h"
+
+struct hodlr_handle_s {
+  std::unique_ptr<hodlr::Engine> eng;
+  std::string err;
+};
+
+namespace {
+
+thread_local std::string g_create_err;
+
+int fail(hodlr_handle h, int code, const std::string& msg) {
+  if (h)
+    h->err = msg;
+  else
+    g_create_err = msg;
+  return code;
+}
+
+template <class F>
+int guard(hodlr_handle h, F&& f) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle");
+  try {
+    f();
+    h->err.clear();
+    return HODLR_OK;
+  } catch (const std::invalid_argument& e) {
+    return fail(h, HODLR_ERR_ARG, e.what());
+  } catch (const hodlr::HipError& e) {
+    const std::string m = e.what();
+    if (m.find("hipMalloc") != std::string::npos || m.find("out of memory") != std::string::npos)
+      return fail(h, HODLR_ERR_NOMEM, m);
+    return fail(h, HODLR_ERR_DEVICE, m);
+  } catch (const std::bad_alloc&) {
+    return fail(h, HODLR_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(h, HODLR_ERR_STATE, e.what());
+  }
+}
+
+int check_device() {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return -1;
+  return 0;
+}
+
+int make_handle(hodlr_handle* h, std::vector<hodlr::InstanceSpec>&& specs) {
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  auto* hh = new (std::nothrow) hodlr_handle_s();
+  if (!hh) return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
+  try {
+    hh->eng.reset(new hodlr::Engine(std::move(specs)));
+  } catch (const std::exception& e) {
+    delete hh;
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  }
+  *h = hh;
+  return HODLR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int leaf) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (n <= 0 || leaf <= 0 || !k || !k->points)
+    return fail(nullptr, HODLR_ERR_ARG, "n, leaf must be positive and points non-null");
+  if (k->dim < 1 || k->dim > 3) return fail(nullptr, HODLR_ERR_ARG, "dim must be 1, 2 or 3");
+  if (k->family < 0 || k->family > HODLR_KERNEL_EXPONENTIAL)
+    return fail(nullptr, HODLR_ERR_ARG, "unknown kernel family");
+  if (!(k->length_scale > 0.0)) return fail(nullptr, HODLR_ERR_ARG, "length_scale must be > 0");
+  if (k->diag_mode != HODLR_DIAG_ADD_SHIFT && k->diag_mode != HODLR_DIAG_REPLACE_VALUE)
+    return fail(nullptr, HODLR_ERR_ARG, "unknown diag_mode");
+  std::vector<hodlr::InstanceSpec> specs(1);
+  hodlr::InstanceSpec& s = specs[0];
+  s.n = n;
+  s.leaf = leaf;
+  s.kind = 0;
+  s.xs.resize(n);
+  if (k->dim > 1) s.ys.resize(n);
+  if (k->dim > 2) s.zs.resize(n);
+  for (int i = 0; i < n; i++) {
+    s.xs[i] = k->points[(size_t)i * k->dim];
+    if (k->dim > 1) s.ys[i] = k->points[(size_t)i * k->dim + 1];
+    if (k->dim > 2) s.zs[i] = k->points[(size_t)i * k->dim + 2];
+  }
+  hodlr_kernel_params_init(&s.kp, k->family, k->dim, k->length_scale, k->diag, k->diag_mode,
+                           nullptr, nullptr, nullptr);
+  return make_handle(h, std::move(specs));
+}
+
+int hodlr_create_kernel_callback(hodlr_handle* h, int n, double (*fn)(int, int, void*), void* data,
+                                 int leaf) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (n <= 0 || leaf <= 0 || !fn) return fail(nullptr, HODLR_ERR_ARG, "bad arguments");
+  std::vector<hodlr::InstanceSpec> specs(1);
+  hodlr::InstanceSpec& s = specs[0];
+  s.n = n;
+  s.leaf = leaf;
+  s.kind = 1;
+  try {
+    s.dense.resize((size_t)n * n);
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, HODLR_ERR_NOMEM, "host allocation of the dense copy failed");
+  }
+  for (int j = 0; j < n; j++)  // column-major, as kernelMatrix::block (kernel.cpp:19-25)
+    for (int i = 0; i < n; i++) s.dense[(size_t)i + (size_t)j * n] = fn(i, j, data);
+  return make_handle(h, std::move(specs));
+}
+
+int hodlr_create_dense(hodlr_handle* h, int n, const double* A, int lda, int leaf, int lr_method) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (n <= 0 || leaf <= 0 || !A || lda < n) return fail(nullptr, HODLR_ERR_ARG, "bad arguments");
+  if (lr_method != HODLR_LR_ACA && lr_method != HODLR_LR_SVD)
+    return fail(nullptr, HODLR_ERR_ARG, "unknown low-rank method");
+  std::vector<hodlr::InstanceSpec> specs(1);
+  hodlr::InstanceSpec& s = specs[0];
+  s.n = n;
+  s.leaf = leaf;
+  s.kind = 1;
+  try {
+    s.dense.resize((size_t)n * n);
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, HODLR_ERR_NOMEM, "host allocation of the dense copy failed");
+  }
+  for (int j = 0; j < n; j++)
+    std::memcpy(s.dense.data() + (size_t)j * n, A + (size_t)j * lda, (size_t)n * sizeof(double));
+  int rc = make_handle(h, std::move(specs));
+  if (rc != HODLR_OK) return rc;
+  rc = hodlr_set_lr_method(*h, lr_method);
+  if (rc != HODLR_OK) {
+    g_create_err = (*h)->err;
+    hodlr_destroy(*h);
+    *h = nullptr;
+  }
+  return rc;
+}
+
+int hodlr_destroy(hodlr_handle h) {
+  if (!h) return HODLR_ERR_ARG;
+  delete h;
+  return HODLR_OK;
+}
+
+int hodlr_set_lr_method(hodlr_handle h, int m) {
+  return guard(h, [&] { h->eng->set_lr_method(m); });
+}
+int hodlr_set_tolerance(hodlr_handle h, double t) {
+  return guard(h, [&] { h->eng->set_tolerance(t); });
+}
+int hodlr_set_min_pivot(hodlr_handle h, double p) {
+  return guard(h, [&] { h->eng->set_min_pivot(p); });
+}
+int hodlr_set_precision(hodlr_handle h, int p) {
+  return guard(h, [&] { h->eng->set_precision(p); });
+}
+int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
+  return guard(h, [&] { h->eng->set_refinement(target, max_iters); });
+}
+
+int hodlr_compress(hodlr_handle h) {
+  return guard(h, [&] { h->eng->compress(); });
+}
+int hodlr_factorize(hodlr_handle h, int which) {
+  return guard(h, [&] { h->eng->factorize(which); });
+}
+int hodlr_solve_sm(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx) {
+  return guard(h, [&] {
+    if (!B || !X || nrhs < 0) throw std::invalid_argument("bad arguments");
+    h->eng->solve_host(HODLR_FACTOR_SM, B, ldb, nrhs, X, ldx);
+  });
+}
+int hodlr_solve_lu(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx) {
+  return guard(h, [&] {
+    if (!B || !X || nrhs < 0) throw std::invalid_argument("bad arguments");
+    h->eng->solve_host(HODLR_FACTOR_LU, B, ldb, nrhs, X, ldx);
+  });
+}
+int hodlr_solve_device(hodlr_handle h, int which, const double* dB, int ldb, int nrhs, double* dX,
+                       int ldx) {
+  return guard(h, [&] {
+    if (!dB || !dX || nrhs < 0) throw std::invalid_argument("bad arguments");
+    h->eng->solve_device(which, dB, ldb, nrhs, dX, ldx);
+  });
+}
+int hodlr_logabsdet(hodlr_handle h, double* out) {
+  return guard(h, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    *out = h->eng->logabsdet(nullptr);
+  });
+}
+int hodlr_determinant(hodlr_handle h, double* out) {
+  return guard(h, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    double p = 0;
+    h->eng->logabsdet(&p);
+    *out = p;
+  });
+}
+int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, int ldy) {
+  return guard(h, [&] {
+    const int n = h->eng->rows();
+    if (!X || !Y || nrhs < 0 || ldx < n || ldy < n) throw std::invalid_argument("bad arguments");
+    if (nrhs == 0) return;
+    hodlr::DBuf<double> dx, dy;
+    dx.alloc((size_t)n * nrhs);
+    dy.alloc((size_t)n * nrhs);
+    HODLR_HIP_CHECK(hipMemcpy2D(dx.p, (size_t)n * 8, X, (size_t)ldx * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyHostToDevice));
+    h->eng->matvec_device(dx.p, n, nrhs, dy.p, n);
+    HODLR_HIP_CHECK(hipMemcpy2D(Y, (size_t)ldy * 8, dy.p, (size_t)n * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyDeviceToHost));
+  });
+}
+
+int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
+  return guard(h, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    std::memset(out, 0, sizeof(*out));
+    const hodlr::Engine& e = *h->eng;
+    out->num_levels = e.num_levels();
+    out->num_nodes = (int)e.nodes().size();
+    out->num_leaves = (int)e.leaves().size();
+    const auto& s = e.stats();
+    out->aca_iterations = s.aca_iterations;
+    out->compress_time = s.compress_time;
+    out->factor_time = s.factor_time;
+    out->solve_time = s.solve_time;
+    out->refine_time = s.refine_time;
+    out->refine_iterations = s.refine_iterations;
+    out->last_residual = s.last_residual;
+    std::vector<int> cnt(64, 0);
+    for (auto& nd : e.nodes()) {
+      if (nd.level >= 64) continue;
+      out->max_rank[nd.level] = std::max(out->max_rank[nd.level], std::max(nd.kt, nd.kb));
+      out->mean_rank[nd.level] += nd.kt + nd.kb;
+      cnt[nd.level] += 2;
+    }
+    for (int l = 0; l < 64; l++)
+      if (cnt[l]) out->mean_rank[l] /= cnt[l];
+    out->device_bytes = e.device_bytes();
+  });
+}
+
+int hodlr_rows(hodlr_handle h) { return h ? h->eng->rows() : HODLR_ERR_ARG; }
+
+int hodlr_num_nodes(hodlr_handle h) { return h ? (int)h->eng->nodes().size() : HODLR_ERR_ARG; }
+
+int hodlr_get_nodes(hodlr_handle h, int* level, int* min_i, int* split, int* max_i, int* kt,
+                    int* kb) {
+  return guard(h, [&] {
+    const auto& nd = h->eng->nodes();
+    for (size_t i = 0; i < nd.size(); i++) {
+      if (level) level[i] = nd[i].level;
+      if (min_i) min_i[i] = nd[i].min_i;
+      if (split) split[i] = nd[i].split;
+      if (max_i) max_i[i] = nd[i].max_i;
+      if (kt) kt[i] = nd[i].kt;
+      if (kb) kb[i] = nd[i].kb;
+    }
+  });
+}
+
+int hodlr_get_factor(hodlr_handle h, int idx, int which, double* out) {
+  int rank = -1;
+  int rc = guard(h, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    h->eng->get_factor(idx, which, out);
+    const auto& nd = h->eng->nodes()[idx];
+    rank = which < 2 ? nd.kt : nd.kb;
+  });
+  return rc == HODLR_OK ? rank : rc;
+}
+
+const char* hodlr_last_error(hodlr_handle h) {
+  return h ? h->err.c_str() : g_create_err.c_str();
+}
+
+const char* hodlr_version(void) { return "dense_hodlr_amd 0.1 (gfx950)"; }
+
+}  // extern "C"

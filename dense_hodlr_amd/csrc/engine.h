@@ -1,0 +1,215 @@
+/*
+ * Host orchestration of the HODLR engine.
+ *
+ * A forest of one or more independent instances (each with its own tree and
+ * entry source) shares level-batched launches: every kernel processes all
+ * blocks, nodes or leaves of one tree level of every instance at once.
+ *
+ * Device layout (column-major FP64, leading dimension Ntot = total rows):
+ *  U[d], V[d]  level panels from compression.  For a node at level d, rows of
+ *              its top half hold U_t (top block rows) and V_b (bottom block
+ *              columns); rows of its bottom half hold U_b and V_t.
+ *  W (per factorization)  Ntot x Ktot: the U^SM columns of every level side
+ *              by side, level d at columns [off[d], off[d] + K[d]).  The
+ *              recSM factorization overwrites W[rows(D), 0:off[d]] by
+ *              D^-1 W[rows(D), 0:off[d]] for every node or leaf D of level d,
+ *              bottom-up (src/HODLR_Matrix.cpp:537-592).
+ *  leaves      s x s blocks, original, LU-factored and inverted copies.
+ *  node C      (kt+kb)^2 capacitance (recSM) or Schur (recLU) matrices, LU.
+ */
+#ifndef DENSE_HODLR_ENGINE_H
+#define DENSE_HODLR_ENGINE_H
+
+#include <deque>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "aca.h"
+#include "common.h"
+#include "dense.h"
+#include "gemm.h"
+
+namespace hodlr {
+
+template <class T>
+struct DBuf {
+  T* p = nullptr;
+  size_t n = 0;
+  DBuf() = default;
+  DBuf(const DBuf&) = delete;
+  DBuf& operator=(const DBuf&) = delete;
+  DBuf(DBuf&& o) noexcept : p(o.p), n(o.n) {
+    o.p = nullptr;
+    o.n = 0;
+  }
+  DBuf& operator=(DBuf&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p;
+      n = o.n;
+      o.p = nullptr;
+      o.n = 0;
+    }
+    return *this;
+  }
+  ~DBuf() { release(); }
+  void alloc(size_t cnt) {
+    release();
+    if (cnt) {
+      hipError_t e = hipMalloc((void**)&p, cnt * sizeof(T));
+      if (e != hipSuccess) {
+        p = nullptr;
+        throw HipError(std::string("hipMalloc of ") + std::to_string(cnt * sizeof(T)) +
+                       " bytes failed: " + hipGetErrorString(e));
+      }
+    }
+    n = cnt;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  size_t bytes() const { return n * sizeof(T); }
+};
+
+/* Bump allocator for per-phase descriptor arrays (device copies of small host
+ * vectors).  Host vectors are kept until reset() so asynchronous copies from
+ * them stay valid. */
+class DescArena {
+ public:
+  template <class T>
+  T* push(const std::vector<T>& v, hipStream_t s);
+  void reset();
+  size_t bytes() const;
+
+ private:
+  struct Chunk {
+    DBuf<char> buf;
+    size_t used = 0;
+  };
+  std::deque<Chunk> chunks_;
+  std::deque<std::vector<char>> keep_;
+};
+
+struct InstanceSpec {
+  int n = 0;
+  int leaf = 30;
+  int kind = 0;  // 0 kernel, 1 dense
+  HodlrKernelParams kp{};
+  std::vector<double> xs, ys, zs;  // coordinates (kernel)
+  std::vector<double> dense;       // n x n (dense)
+};
+
+struct TreeNodeH {
+  int inst, level;
+  int min_i, split, max_i;  // instance-local
+  int row0;                 // global row of min_i
+  int kt = 0, kb = 0;
+};
+
+struct LeafH {
+  int inst, level, min_i, max_i, row0, s;
+  size_t mat_off;  // offset of the s x s block in the leaf buffers
+  size_t piv_off;
+};
+
+struct Factorization {
+  bool done = false;
+  DBuf<double> W;
+  DBuf<double> C;
+  DBuf<int> piv;
+  std::vector<size_t> c_off;   // per node (global node order)
+  std::vector<size_t> p_off;
+};
+
+class Engine {
+ public:
+  explicit Engine(std::vector<InstanceSpec> specs);
+  ~Engine();
+
+  // settings
+  void set_lr_method(int m);
+  void set_tolerance(double t);
+  void set_min_pivot(double p);
+  void set_precision(int p);
+  void set_refinement(double target, int max_iters);
+  void set_initial_rank_capacity(int cap) { init_cap_ = cap; }
+
+  // compute
+  void compress();
+  void factorize(int which);
+  void solve_device(int which, const double* dB, int ldb, int nrhs, double* dX, int ldx);
+  void solve_host(int which, const double* B, int ldb, int nrhs, double* X, int ldx);
+  double logabsdet(double* det_prod);
+  void matvec_device(const double* dX, int ldx, int nrhs, double* dY, int ldy);
+
+  // inspection
+  int rows() const { return ntot_; }
+  int num_levels() const { return nlev_; }
+  const std::vector<TreeNodeH>& nodes() const { return nodes_; }
+  const std::vector<LeafH>& leaves() const { return leaves_; }
+  void get_factor(int idx, int which, double* host_out);
+  struct Stats {
+    int aca_iterations = 0;
+    double compress_time = 0, factor_time = 0, solve_time = 0, refine_time = 0;
+    int refine_iterations = 0;
+    double last_residual = 0;
+  };
+  const Stats& stats() const { return stats_; }
+  long long device_bytes() const;
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void build_trees();
+  void upload_sources();
+  void reset_attributes();
+  void aca_compress();
+  void assemble_leaves();
+  void factor_leaves();
+  void run_gemm(std::vector<GemmDesc>& descs);
+  void ensure_ws(size_t doubles);
+  void lu_solve_batch(std::vector<SolveDesc>& descs, int max_n);
+
+  std::vector<InstanceSpec> specs_;
+  int ntot_ = 0, nlev_ = 0;
+  std::vector<int> inst_row0_;
+  std::vector<TreeNodeH> nodes_;               // level order, left to right
+  std::vector<std::vector<int>> level_nodes_;  // node ids per level
+  std::vector<LeafH> leaves_;
+  std::vector<std::vector<int>> level_leaves_;
+
+  int lr_method_ = 0;
+  double tol_ = 1e-6;   // HODLR_Matrix.cpp:7
+  int min_pivot_ = 0;
+  int precision_ = 0;
+  double refine_target_ = 1e-10;
+  int refine_max_ = 30;
+  int init_cap_ = 0;
+
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+  DescArena arena_;
+  DBuf<EntrySource> d_srcs_;
+  std::vector<DBuf<double>> d_coords_;
+  std::vector<DBuf<double>> d_dense_;
+  std::vector<EntrySource> h_srcs_;
+
+  bool compressed_ = false;
+  bool leaves_factored_ = false;
+  std::vector<DBuf<double>> U_, V_;
+  std::vector<int> K_, off_;
+  int ktot_ = 0;
+  DBuf<double> Lorig_, Lfac_, Linv_;
+  DBuf<int> Lpiv_;
+  size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;
+  Factorization fac_[2];
+  DBuf<double> ws_, tmp_, tmp2_;
+  DBuf<double> xw_;
+  Stats stats_;
+};
+
+}  // namespace hodlr
+
+#endif

@@ -1,0 +1,185 @@
+"""Host-side mirror of the reference's HODLR_Matrix (include/HODLR_Matrix.hpp:38-413).
+
+Same method names and argument meaning as the reference class for the
+kernel/dense constructors, set_LRMethod / set_LRTolerance / set_minPivot,
+recSM_Solve, recLU_Solve, recLU_Compute, logAbsDeterminant and determinant.
+Where the reference prints "Error! ..." and exits, or asserts, this class
+raises HodlrError (or ValueError for argument checks made on the host).
+Everything runs through the C ABI of libdense_hodlr_amd.so on the GPU.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import HodlrError, check, lib
+
+
+def _dptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _as_f64_2d(B, n):
+    B = np.asarray(B, dtype=np.float64)
+    if B.ndim == 1:
+        B = B.reshape(-1, 1)
+    if B.shape[0] != n:
+        # the reference asserts input_RHS.rows() == matrixSize (HODLR_Matrix.cpp:806-807)
+        raise ValueError(f"right-hand side has {B.shape[0]} rows, matrix has {n}")
+    return np.asfortranarray(B)
+
+
+class HODLR_Matrix:
+    """HODLR matrix on one MI355X.
+
+    HODLR_Matrix(matrix, sizeThreshold=30, LR_Method="partialPiv_ACA") is the
+    dense constructor (HODLR_Matrix.hpp:76); use from_kernel() for the typed
+    kernel constructor and from_callback() for the host-callback one
+    (HODLR_Matrix.hpp:162).
+    """
+
+    def __init__(self, matrix=None, sizeThreshold: int = 30, LR_Method: str = "partialPiv_ACA",
+                 *, _handle=None, _n=None):
+        self._h = None
+        if _handle is not None:
+            self._h = _handle
+            self._n = _n
+            return
+        A = np.asfortranarray(np.asarray(matrix, dtype=np.float64))
+        if A.ndim != 2 or A.shape[0] != A.shape[1]:
+            # HODLR_Matrix.cpp:92: assert(isSquareMatrix == true)
+            raise ValueError("the HODLR matrix must be square")
+        if LR_Method not in _lib.LR_METHODS:
+            raise ValueError(f"Invalid low-rank approximation scheme ({LR_Method})")
+        h = C.c_void_p()
+        check(lib().hodlr_create_dense(C.byref(h), A.shape[0], _dptr(A), A.shape[0],
+                                       int(sizeThreshold), _lib.LR_METHODS[LR_Method]))
+        self._h = h
+        self._n = A.shape[0]
+
+    @classmethod
+    def from_kernel(cls, points, kernel: str = "gaussian", length_scale: float = 1.0,
+                    diag: float = 0.0, diag_mode: str = "add", sizeThreshold: int = 30):
+        pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
+        if pts.ndim == 1:
+            pts = pts.reshape(-1, 1)
+        n, dim = pts.shape
+        kd = _lib.KernelDesc(_lib.KERNELS[kernel], dim, _dptr(pts), float(length_scale),
+                             float(diag), {"add": 0, "replace": 1}[diag_mode])
+        h = C.c_void_p()
+        check(lib().hodlr_create_kernel(C.byref(h), n, C.byref(kd), int(sizeThreshold)))
+        return cls(_handle=h, _n=n)
+
+    @classmethod
+    def from_callback(cls, n: int, fn, data=None, sizeThreshold: int = 30):
+        """fn(i, j) -> float, evaluated on the host for every entry."""
+        cb = _lib.CALLBACK(lambda i, j, _p: float(fn(i, j)))
+        h = C.c_void_p()
+        check(lib().hodlr_create_kernel_callback(C.byref(h), int(n), cb, None, int(sizeThreshold)))
+        return cls(_handle=h, _n=n)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and _lib._lib is not None:
+            _lib._lib.hodlr_destroy(self._h)
+            self._h = None
+
+    # ---- attribute modification (HODLR_Matrix.hpp:222-230) -------------
+    def set_LRTolerance(self, tolerance: float):
+        check(lib().hodlr_set_tolerance(self._h, float(tolerance)), self._h)
+
+    def set_minPivot(self, minPivot: float):
+        check(lib().hodlr_set_min_pivot(self._h, float(minPivot)), self._h)
+
+    def set_LRMethod(self, LR_Method: str):
+        if LR_Method not in _lib.LR_METHODS:
+            raise ValueError(f"Invalid low-rank approximation scheme ({LR_Method})")
+        check(lib().hodlr_set_lr_method(self._h, _lib.LR_METHODS[LR_Method]), self._h)
+
+    def set_precision(self, precision: str):
+        check(lib().hodlr_set_precision(self._h, _lib.PRECISIONS[precision]), self._h)
+
+    def set_refinement(self, target: float = 1e-10, max_iters: int = 30):
+        check(lib().hodlr_set_refinement(self._h, float(target), int(max_iters)), self._h)
+
+    # ---- compute ---------------------------------------------------------
+    def storeLRinTree(self):
+        check(lib().hodlr_compress(self._h), self._h)
+
+    def recSM_Factorize(self):
+        check(lib().hodlr_factorize(self._h, _lib.FACTOR_SM), self._h)
+
+    def recLU_Compute(self):
+        check(lib().hodlr_factorize(self._h, _lib.FACTOR_LU), self._h)
+
+    def _solve(self, fn, B):
+        B = _as_f64_2d(B, self._n)
+        X = np.zeros_like(B, order="F")
+        check(fn(self._h, _dptr(B), self._n, B.shape[1], _dptr(X), self._n), self._h)
+        return X
+
+    def recSM_Solve(self, input_RHS):
+        return self._solve(lib().hodlr_solve_sm, input_RHS)
+
+    def recLU_Solve(self, input_RHS):
+        return self._solve(lib().hodlr_solve_lu, input_RHS)
+
+    def logAbsDeterminant(self) -> float:
+        out = C.c_double()
+        check(lib().hodlr_logabsdet(self._h, C.byref(out)), self._h)
+        return out.value
+
+    def determinant(self) -> float:
+        out = C.c_double()
+        check(lib().hodlr_determinant(self._h, C.byref(out)), self._h)
+        return out.value
+
+    def matvec(self, X):
+        """A_H X with the compressed FP64 operator."""
+        X = _as_f64_2d(X, self._n)
+        Y = np.zeros_like(X, order="F")
+        check(lib().hodlr_matvec(self._h, _dptr(X), self._n, X.shape[1], _dptr(Y), self._n), self._h)
+        return Y
+
+    # ---- accessors -------------------------------------------------------
+    def rows(self) -> int:
+        return self._n
+
+    def cols(self) -> int:
+        return self._n
+
+    def stats(self) -> dict:
+        s = _lib.Stats()
+        check(lib().hodlr_get_stats(self._h, C.byref(s)), self._h)
+        L = s.num_levels
+        return dict(num_levels=L, num_nodes=s.num_nodes, num_leaves=s.num_leaves,
+                    aca_iterations=s.aca_iterations, compress_time=s.compress_time,
+                    factor_time=s.factor_time, solve_time=s.solve_time,
+                    refine_time=s.refine_time, refine_iterations=s.refine_iterations,
+                    last_residual=s.last_residual, max_rank=list(s.max_rank[:L]),
+                    mean_rank=list(s.mean_rank[:L]), device_bytes=s.device_bytes)
+
+    def get_LR_ComputationTime(self) -> float:
+        return self.stats()["compress_time"]
+
+    def nodes(self):
+        """(level, min_i, split, max_i, rank_top, rank_bottom) per non-leaf node."""
+        L = lib()
+        c = L.hodlr_num_nodes(self._h)
+        arrs = [np.zeros(c, dtype=np.int32) for _ in range(6)]
+        check(L.hodlr_get_nodes(self._h, *[a.ctypes.data_as(C.POINTER(C.c_int)) for a in arrs]), self._h)
+        return np.stack(arrs, axis=1)
+
+    def factor(self, idx: int, which: int):
+        """Low-rank factor of node idx: 0 top U, 1 top V, 2 bottom U, 3 bottom V."""
+        nd = self.nodes()[idx]
+        lvl, lo, sp, hi, kt, kb = (int(v) for v in nd)
+        rows = {0: sp - lo + 1, 1: hi - sp, 2: hi - sp, 3: sp - lo + 1}[which]
+        k = kt if which < 2 else kb
+        out = np.zeros((rows, k), order="F")
+        check(lib().hodlr_get_factor(self._h, idx, which, _dptr(out)), self._h)
+        return out
+
+
+__all__ = ["HODLR_Matrix", "HodlrError"]

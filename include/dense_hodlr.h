@@ -1,0 +1,181 @@
+/*
+ * dense_hodlr.h -- C ABI of the MI355X HODLR engine.
+ *
+ * Drop-in for the HODLR_Matrix path of the reference (amiraa127/Dense_HODLR,
+ * include/HODLR_Matrix.hpp:38-413): kernel and dense constructors, the choice
+ * of partial-pivot ACA or SVD compression, recSM_Solve, recLU_Solve and the
+ * log-determinant.  Plain pointers and sizes only.  Matrices are column-major
+ * FP64 with an explicit leading dimension, as Eigen::MatrixXd is.
+ *
+ * Conventions
+ *  - Every call returns HODLR_OK (0) or a negative status; the message of the
+ *    last failure is available from hodlr_last_error().  The reference prints
+ *    "Error! ..." and calls exit(EXIT_FAILURE) (src/HODLR_Matrix.cpp:314-315,
+ *    :418-421, :1249-1252) or asserts (:92, :806-807); this ABI returns codes.
+ *  - Host pointers are read or written before the call returns.  The handle
+ *    owns its device memory and one HIP stream; calls on one handle must not
+ *    overlap (the reference object is not reentrant either).
+ *  - Compression and factorization are lazy and cached, and setters that
+ *    change them drop the cache, as in the reference (HODLR_Matrix.cpp:257-272,
+ *    :520-534, :1232-1286).
+ */
+#ifndef DENSE_HODLR_H
+#define DENSE_HODLR_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct hodlr_handle_s* hodlr_handle;
+
+enum {
+  HODLR_OK = 0,
+  HODLR_ERR_ARG = -1,      /* invalid argument (reference: assert)            */
+  HODLR_ERR_STATE = -2,    /* call not valid in this state (reference: exit)  */
+  HODLR_ERR_DEVICE = -3,   /* HIP runtime failure or no usable GPU            */
+  HODLR_ERR_NOMEM = -4,    /* device allocation failed                        */
+  HODLR_ERR_NUMERIC = -5,  /* non-finite result                               */
+  HODLR_ERR_UNSUPPORTED = -6
+};
+
+/* Kernel families (K(i,j) = f(|x_i - x_j|) [+ diagonal term]). */
+enum {
+  HODLR_KERNEL_GAUSSIAN = 0,    /* exp(-r^2 / l^2)                         */
+  HODLR_KERNEL_MATERN32 = 1,    /* (1 + sqrt3 r/l) exp(-sqrt3 r/l)         */
+  HODLR_KERNEL_SE = 2,          /* exp(-r^2 / (2 l^2))                     */
+  HODLR_KERNEL_IMQ = 3,         /* 1 / sqrt(1 + r^2/l^2)                   */
+  HODLR_KERNEL_QUADRATIC = 4,   /* 1 + r^2/l^2                             */
+  HODLR_KERNEL_EXPONENTIAL = 5  /* exp(-r/l)                               */
+};
+
+enum { HODLR_DIAG_ADD_SHIFT = 0, HODLR_DIAG_REPLACE_VALUE = 1 };
+
+/* Low-rank method (reference strings "partialPiv_ACA" and "SVD",
+ * src/HODLR_Matrix.cpp:332, :356). */
+enum { HODLR_LR_ACA = 0, HODLR_LR_SVD = 1 };
+
+/* Arithmetic of the factorization. */
+enum { HODLR_PREC_FP64 = 0, HODLR_PREC_MIXED = 1 };
+
+/* Which factorization (recSM: Woodbury perturbI nodes; recLU: block LU). */
+enum { HODLR_FACTOR_SM = 0, HODLR_FACTOR_LU = 1 };
+
+/* Typed kernel descriptor: replaces the host callback
+ * double(*)(int,int,void*) of include/kernel.hpp:27, which cannot run on the
+ * GPU.  points is n*dim doubles, point-major (x0,y0,x1,y1,...), copied at
+ * creation. */
+typedef struct hodlr_kernel_desc {
+  int family;
+  int dim;              /* 1, 2 or 3 */
+  const double* points;
+  double length_scale;
+  double diag;          /* shift added to K(i,i), or its value (diag_mode) */
+  int diag_mode;        /* HODLR_DIAG_ADD_SHIFT or HODLR_DIAG_REPLACE_VALUE */
+} hodlr_kernel_desc;
+
+/* Per-call statistics (replaces the reference's clock() timers and
+ * per-level rank averages, HODLR_Matrix.cpp:260-264, :424-427, :819-823).
+ * Times are wall-clock seconds measured with HIP events. */
+typedef struct hodlr_stats {
+  int num_levels;
+  int num_nodes;          /* non-leaf nodes */
+  int num_leaves;
+  int aca_iterations;     /* ACA iterations of the batched compression */
+  double compress_time;   /* LR_ComputationTime */
+  double factor_time;     /* recSM/recLU factorization time */
+  double solve_time;      /* last solve */
+  double refine_time;     /* mixed precision: refinement part of the solve */
+  int refine_iterations;
+  double last_residual;   /* mixed precision: final relative residual */
+  int max_rank[64];       /* per level */
+  double mean_rank[64];   /* per level */
+  long long device_bytes; /* device memory held by the handle */
+} hodlr_stats;
+
+/* ---- construction ------------------------------------------------------ */
+
+/* HODLR_Matrix(int nRows, int nCols, double(*)(int,int,void*), void*, int leaf)
+ * (include/HODLR_Matrix.hpp:162, src/HODLR_Matrix.cpp:184-192) with a typed
+ * kernel.  leaf is the size threshold (default 30 in the reference). */
+int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* kernel, int leaf);
+
+/* The same constructor with the reference's host callback: entries are
+ * evaluated on the host (n*n calls, as kernelMatrix::block does,
+ * src/kernel.cpp:19-25) into a dense copy that is uploaded. */
+int hodlr_create_kernel_callback(hodlr_handle* h, int n, double (*fn)(int, int, void*),
+                                 void* data, int leaf);
+
+/* HODLR_Matrix(Eigen::MatrixXd&, int leaf = 30, std::string LR = "partialPiv_ACA")
+ * (include/HODLR_Matrix.hpp:76, src/HODLR_Matrix.cpp:89-98): copies A. */
+int hodlr_create_dense(hodlr_handle* h, int n, const double* A, int lda, int leaf,
+                       int lr_method);
+
+/* Releases all resources (~HODLR_Matrix). */
+int hodlr_destroy(hodlr_handle h);
+
+/* ---- settings ---------------------------------------------------------- */
+
+/* set_LRMethod (HODLR_Matrix.hpp:225, HODLR_Matrix.cpp:1277-1286). */
+int hodlr_set_lr_method(hodlr_handle h, int lr_method);
+/* set_LRTolerance (HODLR_Matrix.hpp:222, HODLR_Matrix.cpp:1248-1257); default 1e-6. */
+int hodlr_set_tolerance(hodlr_handle h, double tol);
+/* set_minPivot (HODLR_Matrix.hpp:223, HODLR_Matrix.cpp:1259-1268).  The ACA
+ * parameter is an int in the reference (include/lowRank.hpp:25): the value is
+ * truncated toward zero. */
+int hodlr_set_min_pivot(hodlr_handle h, double min_pivot);
+/* Factorization arithmetic: FP64 (reference) or mixed (FP32 bulk, FP64 leaves,
+ * capacitance systems and residuals, iterative refinement).  No reference
+ * counterpart. */
+int hodlr_set_precision(hodlr_handle h, int precision);
+/* Mixed precision: relative residual the refinement must reach (default 1e-10)
+ * and the maximum number of refinement steps (default 30). */
+int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
+
+/* ---- compute ----------------------------------------------------------- */
+
+/* storeLRinTree (HODLR_Matrix.hpp:210, HODLR_Matrix.cpp:257-272). */
+int hodlr_compress(hodlr_handle h);
+/* recSM_Factorize / recLU_Compute (HODLR_Matrix.hpp:217, .cpp:505-534, :794-803). */
+int hodlr_factorize(hodlr_handle h, int which);
+/* recSM_Solve (HODLR_Matrix.hpp:215, HODLR_Matrix.cpp:805-841): X = A^-1 B,
+ * B is n x nrhs (ldb), X is n x nrhs (ldx); host pointers. */
+int hodlr_solve_sm(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx);
+/* recLU_Solve (HODLR_Matrix.hpp:214, HODLR_Matrix.cpp:757-792). */
+int hodlr_solve_lu(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx);
+/* recSM/recLU solve with device pointers (which = HODLR_FACTOR_SM / _LU). */
+int hodlr_solve_device(hodlr_handle h, int which, const double* dB, int ldb, int nrhs,
+                       double* dX, int ldx);
+/* logAbsDeterminant (HODLR_Matrix.hpp:249, HODLR_Matrix.cpp:1666-1697). */
+int hodlr_logabsdet(hodlr_handle h, double* out);
+/* determinant (HODLR_Matrix.hpp:248): the product of the LU diagonals without
+ * the permutation sign, as the reference computes it (.cpp:1684-1688). */
+int hodlr_determinant(hodlr_handle h, double* out);
+/* Y = A_H X with the compressed FP64 operator (leaves plus U V^T, the
+ * operator that fill_Block reconstructs, HODLR_Matrix.cpp:1404-1480). */
+int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, int ldy);
+
+/* ---- inspection -------------------------------------------------------- */
+
+int hodlr_get_stats(hodlr_handle h, hodlr_stats* out);
+/* rows() / cols() (HODLR_Matrix.hpp:246-247). */
+int hodlr_rows(hodlr_handle h);
+/* Number of non-leaf nodes, and per node (level order, left to right) its
+ * level, index range and the two off-diagonal ranks. */
+int hodlr_num_nodes(hodlr_handle h);
+int hodlr_get_nodes(hodlr_handle h, int* level, int* min_i, int* split, int* max_i, int* rank_top,
+                    int* rank_bottom);
+/* Copy of one low-rank factor of node idx: which 0 top U, 1 top V, 2 bottom
+ * U, 3 bottom V (topOffDiagU/V, bottOffDiagU/V, HODLR_Tree.hpp:41-44); out is
+ * rows x rank column-major.  Returns the rank or a negative status. */
+int hodlr_get_factor(hodlr_handle h, int idx, int which, double* out);
+/* Message of the last failure on this handle (or of the last failed create
+ * when h is NULL). */
+const char* hodlr_last_error(hodlr_handle h);
+/* Library version string. */
+const char* hodlr_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DENSE_HODLR_H */
