@@ -9,15 +9,13 @@
 namespace hodlr {
 
 constexpr int ACA_MAX_LEVELS = 32;
-constexpr int MAX_FIN_CHUNKS = 64;  // Gram columns up to 64 * 256 = 16384
+constexpr int MAX_FIN_CHUNKS = 64;  // ranks up to 64 * 256 = 16384
 
 /* Device panels of the levels being compressed.  W[l], V[l]: ld x cap[l]
- * column-major; G[l]: per block two cap x cap Gram matrices; P[l]: partial
- * slots of (cap + 4) doubles. */
+ * column-major; P[l]: per-workgroup partial slots of (cap + 4) doubles. */
 struct AcaLevels {
   double* W[ACA_MAX_LEVELS];
   double* V[ACA_MAX_LEVELS];
-  double* G[ACA_MAX_LEVELS];
   double* P[ACA_MAX_LEVELS];
   int cap[ACA_MAX_LEVELS];
   int ld;

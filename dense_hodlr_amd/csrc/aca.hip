@@ -5,25 +5,29 @@
  * chooseNextRowCol (:291-298) for every off-diagonal block of a set of tree
  * levels at once.  One ACA iteration of every active block is three launches:
  *
- *   row phase   residual row r = A(i,:) - W(i,:) V^T, written to V(:,k);
- *               per-workgroup argmax |r|, first unchosen column with
- *               |r| > minPivot, sum r^2 and t = V^T A(i,:) partials
+ *   row phase   residual row r = A(i,:) - W(i,:) V^T (kernel row evaluated on
+ *               the fly), written to V(:,k); per-workgroup argmax |r| and the
+ *               first unchosen column with |r| > minPivot
  *   col phase   pick the pivot column j (first max, else the first unchosen
  *               one), residual column c = A(:,j) - V(j,:) W^T, write
- *               w = c / r_j to W(:,k); argmax |c|, first unchosen row,
- *               sum w^2 and u = W^T A(:,j) partials
- *   finalize    Gram columns d = V^T r = t - G_V W(i,:)^T and
- *               e = W^T w = (u - G_W V(j,:)^T) / r_j, the incremental
- *               Frobenius norm (:234-239), epsilon (:241), the next row and
- *               the stopping rule (:149, :246); the last workgroup of a block
- *               to arrive reduces the partials in a fixed order.
+ *               w = c / r_j to W(:,k); argmax |c| and the first unchosen row
+ *   finalize    the incremental Frobenius norm (:234-239), epsilon (:241),
+ *               the next row and the stopping rule (:149, :226-227, :246);
+ *               the last workgroup of a block to arrive does the scalar logic
+ *               on partials reduced in a fixed order.
  *
- * The residual sums run sequentially over l with separately rounded multiply
- * and add (__dmul_rn/__dadd_rn), the order and rounding of the reference's
- * Eigen loop `sum += W(i,l) * V.col(l)`, so a residual entry is bit-identical
- * to the CPU restatement's.  The Gram identity lets the dot products that the
- * Frobenius update needs come out of the same single pass over the V and W
- * panels that the residuals take, so each iteration streams each panel once.
+ * Stopping test, one iteration late.  The norm update for term k needs the
+ * dot products V^T v_k and W^T w_k.  The row and column passes of iteration
+ * k+1 stream every stored column of V and W anyway, v_k and w_k included, so
+ * they accumulate those dot products exactly in the same pass; the finalize
+ * of iteration k+1 first completes epsilon for term k and, if term k ended the
+ * reference's loop, discards iteration k+1's speculative results.  Each
+ * iteration therefore reads each panel once.
+ *
+ * The residual sums run sequentially over l with the product rounded before
+ * the sum (contraction off), the order and rounding of the reference's Eigen
+ * loop `sum += W(i,l) * V.col(l)`, so residual entries and the stored factors
+ * are bit-identical to the CPU restatement's.
  */
 #include "aca.h"
 
@@ -36,7 +40,7 @@ namespace {
 
 constexpr int TPB = 256;   // threads per workgroup
 constexpr int LCH = 16;    // l-chunk of the transposed partial reduction
-constexpr int FIN_L = 256; // Gram entries per finalize workgroup
+constexpr int FIN_L = 256; // dot products per finalize workgroup
 
 __device__ inline void argmax_merge(double& bv, int& bi, double v, int i) {
   // larger value wins; on equal values the lower index wins (first max)
@@ -52,25 +56,21 @@ __device__ inline bool bit_test(const uint32_t* bits, int i) {
 
 __device__ inline void bit_set(uint32_t* bits, int i) { atomicOr(&bits[i >> 5], 1u << (i & 31)); }
 
-/* Workgroup reduction of (max |.|, argmax, first-unchosen index, sum of squares). */
+/* Workgroup reduction of (max |.|, argmax, first-unchosen index). */
 struct PivRed {
   double v;
   int i;
   int first;
-  double sq;
 };
 
 __device__ PivRed wg_reduce_piv(PivRed p, PivRed* sh) {
   const int t = threadIdx.x;
-  // wave-level
   for (int off = 32; off > 0; off >>= 1) {
-    double ov = __shfl_xor(p.v, off);
-    int oi = __shfl_xor(p.i, off);
-    int of = __shfl_xor(p.first, off);
-    double os = __shfl_xor(p.sq, off);
+    const double ov = __shfl_xor(p.v, off);
+    const int oi = __shfl_xor(p.i, off);
+    const int of = __shfl_xor(p.first, off);
     argmax_merge(p.v, p.i, ov, oi);
     if (of >= 0 && (p.first < 0 || of < p.first)) p.first = of;
-    p.sq += os;
   }
   const int lane = t & 63, wave = t >> 6;
   if (lane == 0) sh[wave] = p;
@@ -80,7 +80,6 @@ __device__ PivRed wg_reduce_piv(PivRed p, PivRed* sh) {
     for (int w = 1; w < TPB / 64; w++) {
       argmax_merge(r.v, r.i, sh[w].v, sh[w].i);
       if (sh[w].first >= 0 && (r.first < 0 || sh[w].first < r.first)) r.first = sh[w].first;
-      r.sq += sh[w].sq;
     }
     sh[0] = r;
   }
@@ -91,7 +90,7 @@ __device__ PivRed wg_reduce_piv(PivRed p, PivRed* sh) {
 }
 
 /* Partial slot layout: (cap + 4) doubles per workgroup slot:
- * [0, cap) dot partials, cap: max |.|, cap+1: argmax, cap+2: first, cap+3: sum sq. */
+ * [0, cap) dot partials, cap: max |.|, cap+1: argmax, cap+2: first unchosen. */
 __device__ inline double* slot_ptr(const AcaLevels& L, const AcaBlock& b, int slot) {
   const int cap = L.cap[b.level];
   return L.P[b.level] + (size_t)(b.part_off + slot) * (size_t)(cap + 4);
@@ -106,7 +105,7 @@ __device__ inline void read_piv(const double* s, int cap, double& v, int& i, int
 /* Reduce the row-phase pivot partials of block b; returns the chosen column
  * (>=0), -1 for a row jump (max |r| <= minPivot), -2 for no valid column. */
 __device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32_t* colbits,
-                             int min_pivot, double* rmax_out) {
+                             int min_pivot) {
   const int cap = L.cap[b.level];
   double bv = -1.0;
   int bi = -1, bf = -1;
@@ -117,10 +116,80 @@ __device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32
     argmax_merge(bv, bi, v, i);
     if (f >= 0 && (bf < 0 || f < bf)) bf = f;
   }
-  if (rmax_out) *rmax_out = bv;
   if (bv <= (double)min_pivot) return -1;
   if (!bit_test(colbits, bi)) return bi;
   return bf >= 0 ? bf : -2;
+}
+
+/* s + w * v with the product rounded before the sum (no fused multiply-add):
+ * the arithmetic of the reference's residual loop.  HIP's __dadd_rn /
+ * __dmul_rn are plain operators that the default contraction would fuse. */
+__device__ __forceinline__ double add_mul_rn(double s, double w, double v) {
+#pragma clang fp contract(off)
+  const double prod = w * v;
+  return s + prod;
+}
+
+/* One pass over the k stored columns of panel P (leading dimension ld) for
+ * the JP panel rows idx[q] this thread owns (-1: none):
+ *   RES:  s[q] = sum_l bc[l*ld] * P(idx[q], l), in order of l with separately
+ *         rounded multiply and add (the reference's residual loop);
+ *   dots: out[l] = workgroup sum of sum_q P(idx[q], l) * pv[q]  (l < k).
+ * Full chunks of LCH columns are unrolled so all their loads issue first. */
+template <int JP, bool RES>
+__device__ __forceinline__ void panel_pass(const double* __restrict__ P,
+                                           const double* __restrict__ bc, int ld, int k,
+                                           const int (&idx)[JP], const double (&pv)[JP],
+                                           bool dots, double (&s)[JP], double (*red)[TPB],
+                                           double* out) {
+  const int t = threadIdx.x;
+  for (int l0 = 0; l0 < k; l0 += LCH) {
+    const int lc = min(LCH, k - l0);
+    if (lc == LCH) {
+      double vv[LCH][JP], bb[LCH];
+#pragma unroll
+      for (int ll = 0; ll < LCH; ll++) {
+        if (RES) bb[ll] = bc[(size_t)(l0 + ll) * ld];
+#pragma unroll
+        for (int q = 0; q < JP; q++)
+          vv[ll][q] = idx[q] >= 0 ? P[(size_t)idx[q] + (size_t)(l0 + ll) * ld] : 0.0;
+      }
+#pragma unroll
+      for (int ll = 0; ll < LCH; ll++) {
+        double p = 0.0;
+#pragma unroll
+        for (int q = 0; q < JP; q++) {
+          if (RES) s[q] = add_mul_rn(s[q], bb[ll], vv[ll][q]);
+          p += vv[ll][q] * pv[q];
+        }
+        if (dots) red[ll][t] = p;
+      }
+    } else {
+      for (int ll = 0; ll < lc; ll++) {
+        const double b = RES ? bc[(size_t)(l0 + ll) * ld] : 0.0;
+        double p = 0.0;
+#pragma unroll
+        for (int q = 0; q < JP; q++) {
+          const double v = idx[q] >= 0 ? P[(size_t)idx[q] + (size_t)(l0 + ll) * ld] : 0.0;
+          if (RES) s[q] = add_mul_rn(s[q], b, v);
+          p += v * pv[q];
+        }
+        if (dots) red[ll][t] = p;
+      }
+    }
+    if (dots) {
+      __syncthreads();
+      const int ll = t >> 4, part = t & 15;
+      double sum = 0.0;
+      if (ll < lc) {
+#pragma unroll
+        for (int u = 0; u < 16; u++) sum += red[ll][part * 16 + u];
+      }
+      for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
+      if (part == 0 && ll < lc) out[l0 + ll] = sum;
+      __syncthreads();
+    }
+  }
 }
 
 __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, AcaState* st,
@@ -135,16 +204,17 @@ __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, A
   s.frob2 = 0.0;
   s.eps = 1.0;
   s.iters = 0;
-  s.pad_[0] = s.pad_[1] = s.pad_[2] = 0;
+  s.pending = 0;
+  s.pad_[0] = s.pad_[1] = 0;
   st[b] = s;
   bits[blocks[b].bits_row0] |= 1u;  // row 0 is chosen first (lowRank.cpp:159)
 }
 
-template <class SRC>
+template <class SRC, int JPT>
 __global__ void __launch_bounds__(TPB)
     aca_row_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    const AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
-                   const SRC* __restrict__ srcs, int jpt, int min_pivot) {
+                   const SRC* __restrict__ srcs, int min_pivot) {
   __shared__ double red[LCH][TPB];
   __shared__ PivRed pr_sh[TPB / 64];
   const int bidx = wg2blk[blockIdx.x];
@@ -156,86 +226,55 @@ __global__ void __launch_bounds__(TPB)
   const int ld = L.ld;
   const int k = s.k;
   const int i = s.cur_row;
+  const bool pend = s.pending != 0;
   const double* __restrict__ Wp = L.W[b.level] + b.w_row0;
   double* __restrict__ Vp = L.V[b.level] + b.v_row0;
   const SRC src = srcs[b.inst];
   const uint32_t* colbits = bits + b.bits_col0;
-  const int cpw = TPB * jpt;
-  const int jbase = local * cpw + t;
-
-  constexpr int JMAX = 4;
-  double a[JMAX], sacc[JMAX];
-  int jj[JMAX];
+  const int jbase = local * TPB * JPT + t;
+  double a[JPT], sacc[JPT], pv[JPT];
+  int jj[JPT];
 #pragma unroll
-  for (int q = 0; q < JMAX; q++) {
+  for (int q = 0; q < JPT; q++) {
     jj[q] = jbase + q * TPB;
-    bool ok = q < jpt && jj[q] < b.n;
+    const bool ok = jj[q] < b.n;
     a[q] = ok ? entry_at(src, b.i0 + i, b.j0 + jj[q]) : 0.0;
+    pv[q] = (ok && pend) ? Vp[(size_t)jj[q] + (size_t)(k - 1) * ld] : 0.0;
     if (!ok) jj[q] = -1;
     sacc[q] = 0.0;
   }
   double* slot = slot_ptr(L, b, local);
-  for (int l0 = 0; l0 < k; l0 += LCH) {
-    const int lc = min(LCH, k - l0);
-    for (int ll = 0; ll < lc; ll++) {
-      const int l = l0 + ll;
-      const double w = Wp[(size_t)i + (size_t)l * ld];
-      double p = 0.0;
-#pragma unroll
-      for (int q = 0; q < JMAX; q++) {
-        if (jj[q] >= 0) {
-          const double v = Vp[(size_t)jj[q] + (size_t)l * ld];
-          sacc[q] = __dadd_rn(sacc[q], __dmul_rn(w, v));
-          p += v * a[q];
-        }
-      }
-      red[ll][t] = p;
-    }
-    __syncthreads();
-    {
-      const int ll = t >> 4, part = t & 15;
-      double sum = 0.0;
-      if (ll < lc) {
-#pragma unroll
-        for (int u = 0; u < 16; u++) sum += red[ll][part * 16 + u];
-      }
-      for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-      if (part == 0 && ll < lc) slot[l0 + ll] = sum;
-    }
-    __syncthreads();
-  }
+  // residual row and, for the pending term k-1, the dots V(:,l) . v_{k-1}
+  panel_pass<JPT, true>(Vp, Wp + i, ld, k, jj, pv, pend, sacc, red, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
   p.first = -1;
-  p.sq = 0.0;
 #pragma unroll
-  for (int q = 0; q < JMAX; q++) {
+  for (int q = 0; q < JPT; q++) {
     if (jj[q] >= 0) {
-      const double r = __dadd_rn(a[q], -sacc[q]);
+      const double r = a[q] - sacc[q];  // lowRank.cpp:167
       Vp[(size_t)jj[q] + (size_t)k * ld] = r;
       const double ar = fabs(r);
       argmax_merge(p.v, p.i, ar, jj[q]);
       if (!bit_test(colbits, jj[q]) && ar > (double)min_pivot && (p.first < 0 || jj[q] < p.first))
         p.first = jj[q];
-      p.sq += r * r;
     }
   }
-  PivRed r = wg_reduce_piv(p, pr_sh);
+  const PivRed r = wg_reduce_piv(p, pr_sh);
   if (t == 0) {
     const int cap = L.cap[b.level];
     slot[cap] = r.v;
     slot[cap + 1] = (double)r.i;
     slot[cap + 2] = (double)r.first;
-    slot[cap + 3] = r.sq;
   }
 }
 
-template <class SRC>
+template <class SRC, int IPT>
 __global__ void __launch_bounds__(TPB)
     aca_col_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
-                   const SRC* __restrict__ srcs, int ipt, int min_pivot) {
+                   const SRC* __restrict__ srcs, int min_pivot) {
   __shared__ double red[LCH][TPB];
   __shared__ PivRed pr_sh[TPB / 64];
   const int bidx = wg2blk[blockIdx.x];
@@ -246,83 +285,63 @@ __global__ void __launch_bounds__(TPB)
   const int t = threadIdx.x;
   const int ld = L.ld;
   const int k = s.k;
-  const int j = decide_column(L, b, bits + b.bits_col0, min_pivot, nullptr);
+  const bool pend = s.pending != 0;
+  const int j = decide_column(L, b, bits + b.bits_col0, min_pivot);
   if (local == 0 && t == 0) st[bidx].cur_col = j;
-  if (j < 0) return;
   double* __restrict__ Wp = L.W[b.level] + b.w_row0;
   const double* __restrict__ Vp = L.V[b.level] + b.v_row0;
+  const int ibase = local * TPB * IPT + t;
+  double a[IPT], sacc[IPT], pw[IPT];
+  int ii[IPT];
+  double* slot = slot_ptr(L, b, b.row_nwg + local);
+  if (j < 0) {
+    // no column this iteration; the pending term still needs W(:,l) . w_{k-1}
+    if (!pend) return;
+#pragma unroll
+    for (int q = 0; q < IPT; q++) {
+      ii[q] = ibase + q * TPB;
+      const bool ok = ii[q] < b.m;
+      pw[q] = ok ? Wp[(size_t)ii[q] + (size_t)(k - 1) * ld] : 0.0;
+      if (!ok) ii[q] = -1;
+      sacc[q] = 0.0;
+    }
+    panel_pass<IPT, false>(Wp, nullptr, ld, k, ii, pw, true, sacc, red, slot);
+    return;
+  }
   const SRC src = srcs[b.inst];
   const uint32_t* rowbits = bits + b.bits_row0;
   const double pivot = 1.0 / Vp[(size_t)j + (size_t)k * ld];  // lowRank.cpp:208
-  const int rpw = TPB * ipt;
-  const int ibase = local * rpw + t;
-
-  constexpr int IMAX = 4;
-  double a[IMAX], sacc[IMAX];
-  int ii[IMAX];
 #pragma unroll
-  for (int q = 0; q < IMAX; q++) {
+  for (int q = 0; q < IPT; q++) {
     ii[q] = ibase + q * TPB;
-    bool ok = q < ipt && ii[q] < b.m;
+    const bool ok = ii[q] < b.m;
     a[q] = ok ? entry_at(src, b.i0 + ii[q], b.j0 + j) : 0.0;
+    pw[q] = (ok && pend) ? Wp[(size_t)ii[q] + (size_t)(k - 1) * ld] : 0.0;
     if (!ok) ii[q] = -1;
     sacc[q] = 0.0;
   }
-  double* slot = slot_ptr(L, b, b.row_nwg + local);
-  for (int l0 = 0; l0 < k; l0 += LCH) {
-    const int lc = min(LCH, k - l0);
-    for (int ll = 0; ll < lc; ll++) {
-      const int l = l0 + ll;
-      const double v = Vp[(size_t)j + (size_t)l * ld];
-      double p = 0.0;
-#pragma unroll
-      for (int q = 0; q < IMAX; q++) {
-        if (ii[q] >= 0) {
-          const double w = Wp[(size_t)ii[q] + (size_t)l * ld];
-          sacc[q] = __dadd_rn(sacc[q], __dmul_rn(v, w));
-          p += w * a[q];
-        }
-      }
-      red[ll][t] = p;
-    }
-    __syncthreads();
-    {
-      const int ll = t >> 4, part = t & 15;
-      double sum = 0.0;
-      if (ll < lc) {
-#pragma unroll
-        for (int u = 0; u < 16; u++) sum += red[ll][part * 16 + u];
-      }
-      for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-      if (part == 0 && ll < lc) slot[l0 + ll] = sum;
-    }
-    __syncthreads();
-  }
+  panel_pass<IPT, true>(Wp, Vp + j, ld, k, ii, pw, pend, sacc, red, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
   p.first = -1;
-  p.sq = 0.0;
 #pragma unroll
-  for (int q = 0; q < IMAX; q++) {
+  for (int q = 0; q < IPT; q++) {
     if (ii[q] >= 0) {
-      const double c = __dadd_rn(a[q], -sacc[q]);
-      const double w = __dmul_rn(pivot, c);  // lowRank.cpp:230
-      Wp[(size_t)ii[q] + (size_t)k * ld] = w;
+      const double c = a[q] - sacc[q];  // lowRank.cpp:215
+      Wp[(size_t)ii[q] + (size_t)k * ld] = pivot * c;  // :230
       const double ac = fabs(c);
       argmax_merge(p.v, p.i, ac, ii[q]);
       if (!bit_test(rowbits, ii[q]) && ac > (double)min_pivot && (p.first < 0 || ii[q] < p.first))
         p.first = ii[q];
-      p.sq += w * w;
     }
   }
-  PivRed r = wg_reduce_piv(p, pr_sh);
+  const PivRed r = wg_reduce_piv(p, pr_sh);
   if (t == 0) {
     const int cap = L.cap[b.level];
     slot[cap] = r.v;
     slot[cap + 1] = (double)r.i;
     slot[cap + 2] = (double)r.first;
-    slot[cap + 3] = r.sq;
   }
 }
 
@@ -331,9 +350,7 @@ __global__ void __launch_bounds__(TPB)
                    const int* __restrict__ wg2chunk, const int* __restrict__ nchunk,
                    AcaState* __restrict__ st, uint32_t* __restrict__ bits,
                    double* __restrict__ fin_part, int* __restrict__ counters, double tol,
-                   int min_rank, int max_rank, int min_pivot) {
-  __shared__ double wrow[FIN_L];
-  __shared__ double vrow[FIN_L];
+                   int min_rank, int max_rank) {
   __shared__ double red[TPB / 64];
   __shared__ int last;
   const int bidx = wg2blk[blockIdx.x];
@@ -342,54 +359,22 @@ __global__ void __launch_bounds__(TPB)
   const AcaState s = st[bidx];
   if (s.status != 0) return;
   const int t = threadIdx.x;
-  const int ld = L.ld;
   const int k = s.k;
-  const int i = s.cur_row;
   const int j = s.cur_col;
   const int cap = L.cap[b.level];
-  const double* Wp = L.W[b.level] + b.w_row0;
-  const double* Vp = L.V[b.level] + b.v_row0;
-  double* GV = L.G[b.level] + (size_t)b.lidx * 2 * (size_t)cap * cap;
-  double* GW = GV + (size_t)cap * cap;
+  const bool pend = s.pending != 0;
 
+  // cross term of the pending term k-1 (lowRank.cpp:234-237): sum_{l<k-1} e_l d_l
   double cross = 0.0;
-  if (j >= 0) {
-    const double pivot = 1.0 / Vp[(size_t)j + (size_t)k * ld];
+  if (pend) {
     const int l = chunk * FIN_L + t;
-    double tl = 0.0, ul = 0.0;
-    if (l < k) {
-      for (int w = 0; w < b.row_nwg; w++) tl += slot_ptr(L, b, w)[l];
-      for (int w = 0; w < b.col_nwg; w++) ul += slot_ptr(L, b, b.row_nwg + w)[l];
-    }
-    // d = t - G_V W(i,:)^T,  e = pivot (u - G_W V(j,:)^T)
-    double gd = 0.0, ge = 0.0;
-    for (int l0 = 0; l0 < k; l0 += FIN_L) {
-      const int lc = min(FIN_L, k - l0);
-      __syncthreads();
-      if (t < lc) {
-        wrow[t] = Wp[(size_t)i + (size_t)(l0 + t) * ld];
-        vrow[t] = Vp[(size_t)j + (size_t)(l0 + t) * ld];
-      }
-      __syncthreads();
-      if (l < k) {
-        for (int q = 0; q < lc; q++) {
-          const size_t o = (size_t)l + (size_t)(l0 + q) * cap;
-          gd += GV[o] * wrow[q];
-          ge += GW[o] * vrow[q];
-        }
-      }
-    }
-    if (l < k) {
-      const double d = tl - gd;
-      const double e = pivot * (ul - ge);
-      GV[(size_t)l + (size_t)k * cap] = d;
-      GV[(size_t)k + (size_t)l * cap] = d;
-      GW[(size_t)l + (size_t)k * cap] = e;
-      GW[(size_t)k + (size_t)l * cap] = e;
+    if (l < k - 1) {
+      double d = 0.0, e = 0.0;
+      for (int w = 0; w < b.row_nwg; w++) d += slot_ptr(L, b, w)[l];
+      for (int w = 0; w < b.col_nwg; w++) e += slot_ptr(L, b, b.row_nwg + w)[l];
       cross = d * e;
     }
   }
-  // workgroup sum of the cross-term partial
   for (int off = 32; off > 0; off >>= 1) cross += __shfl_xor(cross, off);
   if ((t & 63) == 0) red[t >> 6] = cross;
   __syncthreads();
@@ -398,7 +383,7 @@ __global__ void __launch_bounds__(TPB)
     for (int w = 0; w < TPB / 64; w++) c += red[w];
     fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = c;
     __threadfence();
-    int old = atomicAdd(&counters[bidx], 1);
+    const int old = atomicAdd(&counters[bidx], 1);
     last = (old == nchunk[bidx] - 1);
   }
   __syncthreads();
@@ -407,10 +392,26 @@ __global__ void __launch_bounds__(TPB)
   counters[bidx] = 0;
   AcaState ns = s;
   ns.iters = s.iters + 1;
+  ns.pending = 0;
+  if (pend) {
+    // complete epsilon of term k-1 (:238-241) and the loop test (:149)
+    double crossSum = 0.0;
+    for (int c = 0; c < nchunk[bidx]; c++) crossSum += fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
+    double nv2 = 0.0, nw2 = 0.0;
+    for (int w = 0; w < b.row_nwg; w++) nv2 += slot_ptr(L, b, w)[k - 1];
+    for (int w = 0; w < b.col_nwg; w++) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
+    ns.frob2 = s.frob2 + 2.0 * crossSum + nw2 * nv2;
+    ns.eps = sqrt(nw2) * sqrt(nv2) / sqrt(ns.frob2);
+    if (!((ns.eps > tol) || (k < min_rank))) {
+      ns.status = 1;  // term k-1 was the last; this iteration is discarded
+      st[bidx] = ns;
+      return;
+    }
+  }
   uint32_t* rowbits = bits + b.bits_row0;
   uint32_t* colbits = bits + b.bits_col0;
   if (j == -1) {
-    // max |r| <= minPivot: continue from the first unchosen row (lowRank.cpp:187-195)
+    // max |r| <= minPivot: continue from the first unchosen row (:187-195)
     int nr = -1;
     const int nw = (b.m + 31) >> 5;
     for (int w = 0; w < nw && nr < 0; w++) {
@@ -443,36 +444,30 @@ __global__ void __launch_bounds__(TPB)
     argmax_merge(cv, ci, v, ii);
     if (f >= 0 && (cf < 0 || f < cf)) cf = f;
   }
-  int next = (ci >= 0 && !bit_test(rowbits, ci)) ? ci : cf;
+  const int next = (ci >= 0 && !bit_test(rowbits, ci)) ? ci : cf;
   if (next < 0) {  // term k is discarded (:226-227)
     ns.status = 1;
     st[bidx] = ns;
     return;
   }
-  double crossSum = 0.0;
-  for (int c = 0; c < nchunk[bidx]; c++) crossSum += fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
-  double nv2 = 0.0, nw2 = 0.0;
-  for (int w = 0; w < b.row_nwg; w++) nv2 += slot_ptr(L, b, w)[cap + 3];
-  for (int w = 0; w < b.col_nwg; w++) nw2 += slot_ptr(L, b, b.row_nwg + w)[cap + 3];
-  GV[(size_t)k + (size_t)k * cap] = nv2;
-  GW[(size_t)k + (size_t)k * cap] = nw2;
-  ns.frob2 = s.frob2 + 2.0 * crossSum + nw2 * nv2;  // :238
-  ns.eps = sqrt(nw2) * sqrt(nv2) / sqrt(ns.frob2);  // :241
+  // term k is stored (:230-231); its epsilon completes next iteration
   ns.k = k + 1;
   ns.cur_row = next;
   bit_set(rowbits, next);
   const int upper = min(b.m, b.n);
-  const bool cont = ((ns.eps > tol) || (ns.k < min_rank)) && (ns.k < upper);
-  if (!cont || ns.k == max_rank)
-    ns.status = 1;
-  else if (ns.k >= cap)
-    ns.status = 2;
+  if (ns.k == max_rank || ns.k >= upper) {
+    ns.status = 1;  // the loop ends whatever epsilon is (:246-247, :149)
+  } else {
+    ns.pending = 1;
+    if (ns.k >= cap) ns.status = 2;  // storage full: the host grows the panels
+  }
   st[bidx] = ns;
 }
 
 /* Post-processing of the returned factors (lowRank.cpp:250-274):
  * rank 0 -> one zero column; rank >= min(m,n)-1 -> the dense block with an
- * identity factor.  One workgroup per block. */
+ * identity factor; otherwise the speculative column k is cleared.  One
+ * workgroup per block. */
 template <class SRC>
 __global__ void __launch_bounds__(TPB)
     aca_post_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, AcaState* __restrict__ st,
@@ -481,6 +476,7 @@ __global__ void __launch_bounds__(TPB)
   const AcaBlock b = blocks[bidx];
   const int k = st[bidx].k;
   const int ld = L.ld;
+  const int cap = L.cap[b.level];
   double* Wp = L.W[b.level] + b.w_row0;
   double* Vp = L.V[b.level] + b.v_row0;
   const int upper = min(b.m, b.n);
@@ -515,6 +511,10 @@ __global__ void __launch_bounds__(TPB)
     }
     return;
   }
+  if (k < cap) {  // column k may hold a discarded iteration's values
+    for (int i = threadIdx.x; i < b.m; i += TPB) Wp[(size_t)i + (size_t)k * ld] = 0.0;
+    for (int jj = threadIdx.x; jj < b.n; jj += TPB) Vp[(size_t)jj + (size_t)k * ld] = 0.0;
+  }
   if (threadIdx.x == 0) ranks[bidx] = k;
 }
 
@@ -526,17 +526,37 @@ void aca_launch_init(const AcaBlock* d_blocks, AcaState* d_state, uint32_t* d_bi
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
+template <int JPT>
+static void launch_row(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
+                       hipStream_t s) {
+  aca_row_kernel<EntrySource, JPT><<<a.n_row_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_row, a.state,
+                                                             a.bits, d_srcs, a.min_pivot);
+}
+
+template <int IPT>
+static void launch_col(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
+                       hipStream_t s) {
+  aca_col_kernel<EntrySource, IPT><<<a.n_col_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_col, a.state,
+                                                             a.bits, d_srcs, a.min_pivot);
+}
+
 void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
                           hipStream_t s) {
-  aca_row_kernel<EntrySource><<<a.n_row_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_row, a.state,
-                                                        a.bits, d_srcs, a.jpt, a.min_pivot);
+  switch (a.jpt) {
+    case 1: launch_row<1>(L, a, d_srcs, s); break;
+    case 2: launch_row<2>(L, a, d_srcs, s); break;
+    default: launch_row<4>(L, a, d_srcs, s);
+  }
   HODLR_HIP_CHECK(hipGetLastError());
-  aca_col_kernel<EntrySource><<<a.n_col_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_col, a.state,
-                                                        a.bits, d_srcs, a.ipt, a.min_pivot);
+  switch (a.ipt) {
+    case 1: launch_col<1>(L, a, d_srcs, s); break;
+    case 2: launch_col<2>(L, a, d_srcs, s); break;
+    default: launch_col<4>(L, a, d_srcs, s);
+  }
   HODLR_HIP_CHECK(hipGetLastError());
-  aca_fin_kernel<<<a.n_fin_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_fin, a.wg2chunk_fin,
-                                             a.nchunk, a.state, a.bits, a.fin_part, a.counters,
-                                             a.tol, a.min_rank, a.max_rank, a.min_pivot);
+  aca_fin_kernel<<<a.n_fin_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_fin, a.wg2chunk_fin, a.nchunk,
+                                             a.state, a.bits, a.fin_part, a.counters, a.tol,
+                                             a.min_rank, a.max_rank);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

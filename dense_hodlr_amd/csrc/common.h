@@ -69,7 +69,8 @@ struct AcaState {
   double frob2; /* incremental ||W V^T||_F^2 */
   double eps;   /* last ||w|| ||v|| / ||W V^T||_F */
   int iters;    /* iterations executed (including row jumps) */
-  int pad_[3];
+  int pending;  /* 1: term k-1 is stored, its epsilon is completed next iteration */
+  int pad_[2];
 };
 
 /* One problem of a grouped GEMM: C = alpha * op(A) * B + beta * C,

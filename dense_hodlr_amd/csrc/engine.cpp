@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 
 namespace hodlr {
@@ -371,7 +372,7 @@ void Engine::aca_compress() {
     }
   }
   std::vector<int> cap(nl, 0);
-  std::vector<DBuf<double>> G(nl), P(nl);
+  std::vector<DBuf<double>> P(nl);
   AcaLevels L{};
   L.ld = ntot_;
   auto alloc_level = [&](int l, int c) {
@@ -385,31 +386,22 @@ void Engine::aca_compress() {
       HODLR_HIP_CHECK(hipMemcpyAsync(u.p, U_[l].p, old * sizeof(double), hipMemcpyDeviceToDevice, stream_));
       HODLR_HIP_CHECK(hipMemcpyAsync(v.p, V_[l].p, old * sizeof(double), hipMemcpyDeviceToDevice, stream_));
     }
-    DBuf<double> g, p;
-    g.alloc((size_t)lvl_count[l] * 2 * c * c);
+    DBuf<double> p;
     p.alloc((size_t)lvl_slots[l] * (c + 4));
-    if (cap[l] > 0 && G[l].p) {
-      std::vector<CopyDesc> cd;
-      const size_t o2 = (size_t)cap[l] * cap[l], n2 = (size_t)c * c;
-      for (int b = 0; b < lvl_count[l] * 2; b++)
-        cd.push_back({G[l].p + b * o2, g.p + b * n2, cap[l], cap[l], cap[l], c});
-      launch_copy(arena_.push(cd, stream_), (int)cd.size(), stream_);
-    }
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
     U_[l] = std::move(u);
     V_[l] = std::move(v);
-    G[l] = std::move(g);
     P[l] = std::move(p);
     cap[l] = c;
     L.W[l] = U_[l].p;
     L.V[l] = V_[l].p;
-    L.G[l] = G[l].p;
     L.P[l] = P[l].p;
     L.cap[l] = c;
   };
   for (int l = 0; l < nl; l++) {
     if (lvl_count[l] == 0) continue;
     int c = init_cap_ > 0 ? init_cap_ : 32;
+    if (const char* e = std::getenv("HODLR_ACA_INIT_CAP")) c = std::max(1, std::atoi(e));
     c = std::max(1, std::min(c, upper_max[l]));
     alloc_level(l, c);
   }
@@ -531,10 +523,7 @@ void Engine::aca_compress() {
   }
   for (int l = 0; l < nl; l++)
     if (need[l] > cap[l]) alloc_level(l, need[l]);
-  for (int l = 0; l < nl; l++) {
-    G[l].release();
-    P[l].release();
-  }
+  for (int l = 0; l < nl; l++) P[l].release();
   aca_launch_post(L, d_blk.p, d_state.p, d_srcs_.p, d_ranks.p, nb, stream_);
   std::vector<int> ranks(nb);
   HODLR_HIP_CHECK(hipMemcpyAsync(ranks.data(), d_ranks.p, nb * sizeof(int), hipMemcpyDeviceToHost, stream_));

@@ -50,17 +50,20 @@ def test_ranks_identical(gauss1d):
 
 
 def test_factors_match_oracle(gauss1d):
+    """Kernel entries and ACA residuals are computed with the same IEEE
+    operations in the same order as the oracle, so with identical pivots the
+    factors are bit-identical."""
     cfg, g, o = gauss1d
     _case("factors gauss1d_4096")
+    bad = []
     for idx in range(len(o.nodes())):
         for which in range(4):
             a, b = g.factor(idx, which), o.factor(idx, which)
             assert a.shape == b.shape
-            scale = max(np.abs(b).max(), 1e-300)
-            # residual rows/columns are bit-identical; only the stored terms
-            # accumulate rounding through the Gram-based norm update (no effect
-            # on the factors themselves)
-            assert np.abs(a - b).max() <= 1e-13 * scale, (idx, which)
+            if not np.array_equal(a, b):
+                cols = np.nonzero(np.any(a != b, axis=0))[0]
+                bad.append((idx, which, int(cols[0]), float(np.abs(a - b).max())))
+    assert not bad, bad[:10]
 
 
 def test_recsm_matches_oracle(gauss1d, oracle):
