@@ -9,7 +9,7 @@
 namespace hodlr {
 
 constexpr int ACA_MAX_LEVELS = 32;
-constexpr int MAX_FIN_CHUNKS = 64;  // ranks up to 64 * 256 = 16384
+constexpr int MAX_FIN_CHUNKS = 256;  // ranks up to 256 * 64 = 16384
 
 /* Device panels of the levels being compressed.  W[l], V[l]: ld x cap[l]
  * column-major; P[l]: per-workgroup partial slots of (cap + 4) doubles. */

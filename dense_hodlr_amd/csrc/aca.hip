@@ -40,7 +40,7 @@ namespace {
 
 constexpr int TPB = 256;   // threads per workgroup
 constexpr int LCH = 16;    // l-chunk of the transposed partial reduction
-constexpr int FIN_L = 256; // dot products per finalize workgroup
+constexpr int FIN_L = 64;  // dot products per finalize workgroup (x4 slot groups)
 
 __device__ inline void argmax_merge(double& bv, int& bi, double v, int i) {
   // larger value wins; on equal values the lower index wins (first max)
@@ -102,23 +102,57 @@ __device__ inline void read_piv(const double* s, int cap, double& v, int& i, int
   f = (int)s[cap + 2];
 }
 
-/* Reduce the row-phase pivot partials of block b; returns the chosen column
- * (>=0), -1 for a row jump (max |r| <= minPivot), -2 for no valid column. */
-__device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32_t* colbits,
-                             int min_pivot) {
+/* Workgroup-parallel reduction of the pivot partials in slots
+ * [s0, s0 + nslot): (max |.|, argmax, first unchosen). */
+__device__ PivRed reduce_slots(const AcaLevels& L, const AcaBlock& b, int s0, int nslot,
+                               PivRed* sh) {
   const int cap = L.cap[b.level];
-  double bv = -1.0;
-  int bi = -1, bf = -1;
-  for (int w = 0; w < b.row_nwg; w++) {
+  PivRed p;
+  p.v = -1.0;
+  p.i = -1;
+  p.first = -1;
+  for (int w = threadIdx.x; w < nslot; w += TPB) {
     double v;
     int i, f;
-    read_piv(slot_ptr(L, b, w), cap, v, i, f);
-    argmax_merge(bv, bi, v, i);
-    if (f >= 0 && (bf < 0 || f < bf)) bf = f;
+    read_piv(slot_ptr(L, b, s0 + w), cap, v, i, f);
+    argmax_merge(p.v, p.i, v, i);
+    if (f >= 0 && (p.first < 0 || f < p.first)) p.first = f;
   }
-  if (bv <= (double)min_pivot) return -1;
-  if (!bit_test(colbits, bi)) return bi;
-  return bf >= 0 ? bf : -2;
+  return wg_reduce_piv(p, sh);
+}
+
+/* The pivot column of this iteration from the row-phase partials: >= 0 the
+ * column, -1 a row jump (max |r| <= minPivot), -2 no admissible column. */
+__device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32_t* colbits,
+                             int min_pivot, PivRed* sh) {
+  const PivRed r = reduce_slots(L, b, 0, b.row_nwg, sh);
+  if (r.v <= (double)min_pivot) return -1;
+  if (!bit_test(colbits, r.i)) return r.i;
+  return r.first >= 0 ? r.first : -2;
+}
+
+/* sum over w = g, g+4, g+8, ... < n of p[w * stride], eight loads in flight */
+__device__ __forceinline__ double sum_strided(const double* __restrict__ p, int g, int n,
+                                              size_t stride) {
+  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  int w = g;
+  for (; w + 28 < n; w += 32) {
+#pragma unroll
+    for (int u = 0; u < 8; u++) acc[u] += p[(size_t)(w + 4 * u) * stride];
+  }
+  for (; w < n; w += 4) acc[0] += p[(size_t)w * stride];
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+}
+
+/* Deterministic workgroup sum (fixed reduction shape). */
+__device__ double wg_sum(double v, double* sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  double r = 0.0;
+  for (int w = 0; w < TPB / 64; w++) r += sh[w];
+  __syncthreads();
+  return r;
 }
 
 /* s + w * v with the product rounded before the sum (no fused multiply-add):
@@ -286,7 +320,7 @@ __global__ void __launch_bounds__(TPB)
   const int ld = L.ld;
   const int k = s.k;
   const bool pend = s.pending != 0;
-  const int j = decide_column(L, b, bits + b.bits_col0, min_pivot);
+  const int j = decide_column(L, b, bits + b.bits_col0, min_pivot, pr_sh);
   if (local == 0 && t == 0) st[bidx].cur_col = j;
   double* __restrict__ Wp = L.W[b.level] + b.w_row0;
   const double* __restrict__ Vp = L.V[b.level] + b.v_row0;
@@ -352,7 +386,10 @@ __global__ void __launch_bounds__(TPB)
                    double* __restrict__ fin_part, int* __restrict__ counters, double tol,
                    int min_rank, int max_rank) {
   __shared__ double red[TPB / 64];
+  __shared__ double part[4][FIN_L];
+  __shared__ PivRed pr_sh[TPB / 64];
   __shared__ int last;
+  __shared__ int first_row;
   const int bidx = wg2blk[blockIdx.x];
   const int chunk = wg2chunk[blockIdx.x];
   const AcaBlock b = blocks[bidx];
@@ -361,45 +398,81 @@ __global__ void __launch_bounds__(TPB)
   const int t = threadIdx.x;
   const int k = s.k;
   const int j = s.cur_col;
-  const int cap = L.cap[b.level];
   const bool pend = s.pending != 0;
 
-  // cross term of the pending term k-1 (lowRank.cpp:234-237): sum_{l<k-1} e_l d_l
+  // cross term of the pending term k-1 (lowRank.cpp:234-237): sum_{l<k-1} e_l d_l.
+  // FIN_L values of l per workgroup, each summed over the slots by 4 groups.
   double cross = 0.0;
   if (pend) {
-    const int l = chunk * FIN_L + t;
+    const int ll = t & (FIN_L - 1), g = t / FIN_L;
+    const int l = chunk * FIN_L + ll;
+    double d = 0.0, e = 0.0;
     if (l < k - 1) {
-      double d = 0.0, e = 0.0;
-      for (int w = 0; w < b.row_nwg; w++) d += slot_ptr(L, b, w)[l];
-      for (int w = 0; w < b.col_nwg; w++) e += slot_ptr(L, b, b.row_nwg + w)[l];
-      cross = d * e;
+      const size_t stride = (size_t)(L.cap[b.level] + 4);
+      const double* pr = slot_ptr(L, b, 0) + l;
+      const double* pc = slot_ptr(L, b, b.row_nwg) + l;
+      d = sum_strided(pr, g, b.row_nwg, stride);
+      e = sum_strided(pc, g, b.col_nwg, stride);
+    }
+    part[g][ll] = d;
+    __syncthreads();
+    if (g == 0) d = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
+    __syncthreads();
+    part[g][ll] = e;
+    __syncthreads();
+    if (g == 0) {
+      e = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
+      cross = (l < k - 1) ? d * e : 0.0;
     }
   }
-  for (int off = 32; off > 0; off >>= 1) cross += __shfl_xor(cross, off);
-  if ((t & 63) == 0) red[t >> 6] = cross;
-  __syncthreads();
+  cross = wg_sum(cross, red);
   if (t == 0) {
-    double c = 0.0;
-    for (int w = 0; w < TPB / 64; w++) c += red[w];
-    fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = c;
+    fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = cross;
     __threadfence();
     const int old = atomicAdd(&counters[bidx], 1);
     last = (old == nchunk[bidx] - 1);
   }
   __syncthreads();
-  if (!last || t != 0) return;
+  if (!last) return;
+  // the last workgroup of the block: all its threads reduce, thread 0 decides
   __threadfence();
+  const int cap = L.cap[b.level];
+  double crossSum = 0.0, nv2 = 0.0, nw2 = 0.0;
+  if (pend) {
+    for (int c = t; c < nchunk[bidx]; c += TPB) crossSum += fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
+    for (int w = t; w < b.row_nwg; w += TPB) nv2 += slot_ptr(L, b, w)[k - 1];
+    for (int w = t; w < b.col_nwg; w += TPB) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
+    crossSum = wg_sum(crossSum, red);
+    nv2 = wg_sum(nv2, red);
+    nw2 = wg_sum(nw2, red);
+  }
+  uint32_t* rowbits = bits + b.bits_row0;
+  uint32_t* colbits = bits + b.bits_col0;
+  PivRed cr;
+  cr.v = -1.0;
+  cr.i = -1;
+  cr.first = -1;
+  if (j >= 0) cr = reduce_slots(L, b, b.row_nwg, b.col_nwg, pr_sh);
+  if (j == -1) {  // first unchosen row for a jump
+    if (t == 0) first_row = 0x7fffffff;
+    __syncthreads();
+    const int nw = (b.m + 31) >> 5;
+    int mine = -1;
+    for (int w = t; w < nw && mine < 0; w += TPB) {
+      uint32_t x = ~rowbits[w];
+      if (w == nw - 1 && (b.m & 31)) x &= (1u << (b.m & 31)) - 1u;
+      if (x) mine = w * 32 + __ffs(x) - 1;
+    }
+    if (mine >= 0) atomicMin(&first_row, mine);
+    __syncthreads();
+  }
+  if (t != 0) return;
   counters[bidx] = 0;
   AcaState ns = s;
   ns.iters = s.iters + 1;
   ns.pending = 0;
   if (pend) {
     // complete epsilon of term k-1 (:238-241) and the loop test (:149)
-    double crossSum = 0.0;
-    for (int c = 0; c < nchunk[bidx]; c++) crossSum += fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
-    double nv2 = 0.0, nw2 = 0.0;
-    for (int w = 0; w < b.row_nwg; w++) nv2 += slot_ptr(L, b, w)[k - 1];
-    for (int w = 0; w < b.col_nwg; w++) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
     ns.frob2 = s.frob2 + 2.0 * crossSum + nw2 * nv2;
     ns.eps = sqrt(nw2) * sqrt(nv2) / sqrt(ns.frob2);
     if (!((ns.eps > tol) || (k < min_rank))) {
@@ -408,18 +481,10 @@ __global__ void __launch_bounds__(TPB)
       return;
     }
   }
-  uint32_t* rowbits = bits + b.bits_row0;
-  uint32_t* colbits = bits + b.bits_col0;
   if (j == -1) {
     // max |r| <= minPivot: continue from the first unchosen row (:187-195)
-    int nr = -1;
-    const int nw = (b.m + 31) >> 5;
-    for (int w = 0; w < nw && nr < 0; w++) {
-      uint32_t x = ~rowbits[w];
-      if (w == nw - 1 && (b.m & 31)) x &= (1u << (b.m & 31)) - 1u;
-      if (x) nr = w * 32 + __ffs(x) - 1;
-    }
-    if (nr < 0) {
+    const int nr = first_row;
+    if (nr == 0x7fffffff) {
       ns.status = 1;
     } else {
       ns.cur_row = nr;
@@ -435,16 +500,7 @@ __global__ void __launch_bounds__(TPB)
   }
   bit_set(colbits, j);  // :206
   // next row (:218-227)
-  double cv = -1.0;
-  int ci = -1, cf = -1;
-  for (int w = 0; w < b.col_nwg; w++) {
-    double v;
-    int ii, f;
-    read_piv(slot_ptr(L, b, b.row_nwg + w), cap, v, ii, f);
-    argmax_merge(cv, ci, v, ii);
-    if (f >= 0 && (cf < 0 || f < cf)) cf = f;
-  }
-  const int next = (ci >= 0 && !bit_test(rowbits, ci)) ? ci : cf;
+  const int next = (cr.i >= 0 && !bit_test(rowbits, cr.i)) ? cr.i : cr.first;
   if (next < 0) {  // term k is discarded (:226-227)
     ns.status = 1;
     st[bidx] = ns;

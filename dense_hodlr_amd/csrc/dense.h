@@ -62,10 +62,33 @@ struct CapDesc {
   int pad_;
 };
 
+/* Triangular solve with a pw x pw diagonal block (pw <= 64) of an LU
+ * factor: unit lower (upper = 0) or upper (upper = 1); B is pw x ncols. */
+struct TrsmDesc {
+  const double* T;
+  double* B;
+  int ldt, ldb;
+  int pw;
+  int upper;
+  int ncols;
+  int chunk0; /* first workgroup (64 columns each) of this problem */
+};
+
+/* Row interchanges piv[0..n) applied in order to B (n x ncols). */
+struct PermDesc {
+  double* B;
+  const int* piv;
+  int ldb;
+  int n;
+  int ncols;
+  int chunk0; /* first workgroup (256 columns each) */
+};
+
+void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, hipStream_t s);
+void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
+void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s);
-void launch_lu(const LuDesc* d, int count, hipStream_t s);
-void launch_lu_solve(const SolveDesc* d, int count, int nwg, int ch, int max_n, hipStream_t s);
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);
 void launch_copy(const CopyDesc* d, int count, hipStream_t s);
 void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod, hipStream_t s);

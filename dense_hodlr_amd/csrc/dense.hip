@@ -21,132 +21,6 @@ __global__ void __launch_bounds__(TPB)
   }
 }
 
-/* Right-looking partial-pivot LU, one workgroup per matrix.  The pivot is
- * the first row of maximum |a_ik| (LAPACK getrf / Eigen PartialPivLU). */
-__global__ void __launch_bounds__(TPB) lu_kernel(const LuDesc* __restrict__ d) {
-  __shared__ double sv[TPB / 64];
-  __shared__ int si[TPB / 64];
-  __shared__ int piv_sh;
-  const LuDesc m = d[blockIdx.x];
-  double* A = m.A;
-  const int n = m.n, ld = m.ld, t = threadIdx.x;
-  for (int k = 0; k < n; k++) {
-    double bv = -1.0;
-    int bi = n;
-    for (int i = k + t; i < n; i += TPB) {
-      const double v = fabs(A[(size_t)i + (size_t)k * ld]);
-      if (v > bv) {
-        bv = v;
-        bi = i;
-      }
-    }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(bv, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) {
-        bv = ov;
-        bi = oi;
-      }
-    }
-    if ((t & 63) == 0) {
-      sv[t >> 6] = bv;
-      si[t >> 6] = bi;
-    }
-    __syncthreads();
-    if (t == 0) {
-      double v = sv[0];
-      int ix = si[0];
-      for (int w = 1; w < TPB / 64; w++)
-        if (sv[w] > v || (sv[w] == v && si[w] < ix)) {
-          v = sv[w];
-          ix = si[w];
-        }
-      piv_sh = ix;
-      m.piv[k] = ix;
-    }
-    __syncthreads();
-    const int p = piv_sh;
-    if (p != k)
-      for (int j = t; j < n; j += TPB) {
-        const double x = A[(size_t)k + (size_t)j * ld];
-        A[(size_t)k + (size_t)j * ld] = A[(size_t)p + (size_t)j * ld];
-        A[(size_t)p + (size_t)j * ld] = x;
-      }
-    __syncthreads();
-    const double pv = A[(size_t)k + (size_t)k * ld];
-    if (pv != 0.0) {
-      const double inv = 1.0 / pv;
-      for (int i = k + 1 + t; i < n; i += TPB) A[(size_t)i + (size_t)k * ld] *= inv;
-    }
-    __syncthreads();
-    const int r = n - k - 1;
-    const size_t tot = (size_t)r * r;
-    for (size_t e = t; e < tot; e += TPB) {
-      const int i = k + 1 + (int)(e % r), j = k + 1 + (int)(e / r);
-      A[(size_t)i + (size_t)j * ld] -= A[(size_t)i + (size_t)k * ld] * A[(size_t)k + (size_t)j * ld];
-    }
-    __syncthreads();
-  }
-}
-
-/* In-place LU solve of a chunk of ch right-hand-side columns, staged in LDS.
- * The workgroup -> (problem, chunk) mapping is a binary search on chunk0. */
-__global__ void __launch_bounds__(TPB)
-    lu_solve_kernel(const SolveDesc* __restrict__ d, int count, int ch) {
-  extern __shared__ double x[];
-  int lo = 0, hi = count - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (d[mid].chunk0 <= (int)blockIdx.x)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const SolveDesc p = d[lo];
-  const int c0 = ((int)blockIdx.x - p.chunk0) * ch;
-  const int nc = min(ch, p.ncols - c0);
-  if (nc <= 0) return;
-  const int n = p.n, t = threadIdx.x;
-  const double* LU = p.LU;
-  const int ldl = p.ld_lu;
-  for (int e = t; e < n * nc; e += TPB) {
-    const int i = e % n, c = e / n;
-    x[i + c * n] = p.B[(size_t)i + (size_t)(c0 + c) * p.ldb];
-  }
-  __syncthreads();
-  if (t < nc)
-    for (int k = 0; k < n; k++) {
-      const int q = p.piv[k];
-      if (q != k) {
-        const double tmp = x[k + t * n];
-        x[k + t * n] = x[q + t * n];
-        x[q + t * n] = tmp;
-      }
-    }
-  __syncthreads();
-  for (int k = 0; k < n - 1; k++) {  // unit lower triangle
-    const int r = n - k - 1;
-    for (int e = t; e < r * nc; e += TPB) {
-      const int i = k + 1 + e % r, c = e / r;
-      x[i + c * n] -= LU[(size_t)i + (size_t)k * ldl] * x[k + c * n];
-    }
-    __syncthreads();
-  }
-  for (int k = n - 1; k >= 0; k--) {  // upper triangle
-    if (t < nc) x[k + t * n] /= LU[(size_t)k + (size_t)k * ldl];
-    __syncthreads();
-    for (int e = t; e < k * nc; e += TPB) {
-      const int i = e % k, c = e / k;
-      x[i + c * n] -= LU[(size_t)i + (size_t)k * ldl] * x[k + c * n];
-    }
-    __syncthreads();
-  }
-  for (int e = t; e < n * nc; e += TPB) {
-    const int i = e % n, c = e / n;
-    p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[i + c * n];
-  }
-}
-
 __global__ void __launch_bounds__(TPB) identity_kernel(const CopyDesc* __restrict__ d) {
   const CopyDesc c = d[blockIdx.y];
   const size_t tot = (size_t)c.m * c.n;
@@ -236,6 +110,145 @@ __global__ void __launch_bounds__(TPB) capacitance_kernel(const CapDesc* __restr
   }
 }
 
+/* Panel step of the blocked right-looking LU: factor columns [p0, p0+pw) of
+ * rows [p0, n) with partial pivoting (first maximum), swapping whole rows so
+ * the trailing columns follow (LAPACK getrf order).  One workgroup per
+ * matrix; matrices with n <= p0 are done. */
+__global__ void __launch_bounds__(TPB) lu_panel_kernel(const LuDesc* __restrict__ d, int p0,
+                                                       int nb) {
+  __shared__ double sv[TPB / 64];
+  __shared__ int si[TPB / 64];
+  __shared__ int piv_sh;
+  const LuDesc m = d[blockIdx.x];
+  const int n = m.n, ld = m.ld, t = threadIdx.x;
+  if (n <= p0) return;
+  double* A = m.A;
+  const int pe = min(n, p0 + nb);
+  for (int c = p0; c < pe; c++) {
+    double bv = -1.0;
+    int bi = n;
+    for (int i = c + t; i < n; i += TPB) {
+      const double v = fabs(A[(size_t)i + (size_t)c * ld]);
+      if (v > bv) {
+        bv = v;
+        bi = i;
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if ((t & 63) == 0) {
+      sv[t >> 6] = bv;
+      si[t >> 6] = bi;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double v = sv[0];
+      int ix = si[0];
+      for (int w = 1; w < TPB / 64; w++)
+        if (sv[w] > v || (sv[w] == v && si[w] < ix)) {
+          v = sv[w];
+          ix = si[w];
+        }
+      piv_sh = ix;
+      m.piv[c] = ix;
+    }
+    __syncthreads();
+    const int p = piv_sh;
+    if (p != c)
+      for (int j = t; j < n; j += TPB) {
+        const double x = A[(size_t)c + (size_t)j * ld];
+        A[(size_t)c + (size_t)j * ld] = A[(size_t)p + (size_t)j * ld];
+        A[(size_t)p + (size_t)j * ld] = x;
+      }
+    __syncthreads();
+    const double pv = A[(size_t)c + (size_t)c * ld];
+    if (pv != 0.0) {
+      const double inv = 1.0 / pv;
+      for (int i = c + 1 + t; i < n; i += TPB) A[(size_t)i + (size_t)c * ld] *= inv;
+    }
+    __syncthreads();
+    const int r = n - c - 1, w = pe - c - 1;  // update the rest of the panel only
+    const size_t tot = (size_t)r * w;
+    for (size_t e = t; e < tot; e += TPB) {
+      const int i = c + 1 + (int)(e % r), j = c + 1 + (int)(e / r);
+      A[(size_t)i + (size_t)j * ld] -= A[(size_t)i + (size_t)c * ld] * A[(size_t)c + (size_t)j * ld];
+    }
+    __syncthreads();
+  }
+}
+
+/* Triangular solve of a pw x pw diagonal block (pw <= 64) on a chunk of 64
+ * right-hand-side columns: unit lower (forward) or upper (backward).  One
+ * wave per chunk, one column per lane, the block and the columns in LDS. */
+__global__ void __launch_bounds__(64) trsm_kernel(const TrsmDesc* __restrict__ d, int count) {
+  __shared__ double T[64][65];
+  __shared__ double X[64][65];
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const TrsmDesc p = d[lo];
+  const int c0 = ((int)blockIdx.x - p.chunk0) * 64;
+  const int t = threadIdx.x, pw = p.pw;
+  for (int e = t; e < pw * pw; e += 64) {
+    const int r = e % pw, c = e / pw;
+    T[r][c] = p.T[(size_t)r + (size_t)c * p.ldt];
+  }
+  const int col = c0 + t;
+  const bool ok = col < p.ncols;
+  for (int r = 0; r < pw; r++) X[r][t] = ok ? p.B[(size_t)r + (size_t)col * p.ldb] : 0.0;
+  __syncthreads();
+  if (!p.upper) {
+    for (int r = 1; r < pw; r++) {
+      double v = X[r][t];
+      for (int q = 0; q < r; q++) v -= T[r][q] * X[q][t];
+      X[r][t] = v;
+    }
+  } else {
+    for (int r = pw - 1; r >= 0; r--) {
+      double v = X[r][t];
+      for (int q = r + 1; q < pw; q++) v -= T[r][q] * X[q][t];
+      X[r][t] = v / T[r][r];
+    }
+  }
+  if (ok)
+    for (int r = 0; r < pw; r++) p.B[(size_t)r + (size_t)col * p.ldb] = X[r][t];
+}
+
+/* Row interchanges of an LU applied to right-hand sides, one column per thread. */
+__global__ void __launch_bounds__(TPB) permute_kernel(const PermDesc* __restrict__ d, int count) {
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const PermDesc p = d[lo];
+  const int col = ((int)blockIdx.x - p.chunk0) * TPB + threadIdx.x;
+  if (col >= p.ncols) return;
+  double* b = p.B + (size_t)col * p.ldb;
+  for (int k = 0; k < p.n; k++) {
+    const int q = p.piv[k];
+    if (q != k) {
+      const double x = b[k];
+      b[k] = b[q];
+      b[q] = x;
+    }
+  }
+}
+
 int grid_x(size_t elems, int ndesc) {
   size_t g = (elems + TPB - 1) / TPB;
   size_t cap = ndesc > 64 ? 16 : 1024;
@@ -254,17 +267,21 @@ void launch_assemble(const AssembleDesc* d, int count, int max_elems, const Entr
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-void launch_lu(const LuDesc* d, int count, hipStream_t s) {
+void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, hipStream_t s) {
   if (count == 0) return;
-  lu_kernel<<<count, TPB, 0, s>>>(d);
+  lu_panel_kernel<<<count, TPB, 0, s>>>(d, p0, nb);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-void launch_lu_solve(const SolveDesc* d, int count, int nwg, int ch, int max_n, hipStream_t s) {
-  if (nwg == 0 || count == 0) return;
-  const size_t lds = (size_t)max_n * ch * sizeof(double);
-  if (lds > 64 * 1024) throw HipError("lu_solve: chunk does not fit in LDS");
-  lu_solve_kernel<<<nwg, TPB, lds, s>>>(d, count, ch);
+void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
+  if (count == 0 || nwg == 0) return;
+  trsm_kernel<<<nwg, 64, 0, s>>>(d, count);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s) {
+  if (count == 0 || nwg == 0) return;
+  permute_kernel<<<nwg, TPB, 0, s>>>(d, count);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

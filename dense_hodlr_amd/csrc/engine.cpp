@@ -252,31 +252,29 @@ long long Engine::device_bytes() const {
 /* helpers                                                             */
 /* ------------------------------------------------------------------ */
 void Engine::ensure_ws(size_t doubles) {
-  if (ws_.n < doubles) ws_.alloc(doubles + doubles / 4);
+  if (ws_.n < doubles) {
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));  // queued kernels may use the old buffer
+    ws_.alloc(doubles + doubles / 4);
+  }
+}
+
+void Engine::grow(DBuf<double>& b, size_t doubles) {
+  if (b.n < doubles) {
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    b.alloc(doubles);
+  }
 }
 
 void Engine::run_gemm(std::vector<GemmDesc>& descs) {
   if (descs.empty()) return;
   std::vector<int> split_ids;
   int ntiles = 0;
-  const size_t ws = gemm_plan(descs, split_ids, ntiles, 2048);
+  const int bm = gemm_tile_m(descs);
+  const size_t ws = gemm_plan(descs, split_ids, ntiles, 2048, bm);
   if (ws) ensure_ws(ws);
   const GemmDesc* d = arena_.push(descs, stream_);
   const int* ids = split_ids.empty() ? nullptr : arena_.push(split_ids, stream_);
-  launch_gemm(d, (int)descs.size(), ntiles, ws_.p, ids, (int)split_ids.size(), stream_);
-}
-
-void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, int max_n) {
-  if (descs.empty()) return;
-  int ch = 32;
-  while (ch > 1 && (size_t)max_n * ch > 8192) ch >>= 1;
-  int nwg = 0;
-  for (auto& d : descs) {
-    d.chunk0 = nwg;
-    nwg += (d.ncols + ch - 1) / ch;
-  }
-  const SolveDesc* dd = arena_.push(descs, stream_);
-  launch_lu_solve(dd, (int)descs.size(), nwg, ch, max_n, stream_);
+  launch_gemm(d, (int)descs.size(), ntiles, ws_.p, ids, (int)split_ids.size(), bm, stream_);
 }
 
 static GemmDesc gemm_desc(int M, int N, int K, int ta, const double* A, int lda, const double* B,
@@ -297,6 +295,106 @@ static GemmDesc gemm_desc(int M, int N, int K, int ta, const double* A, int lda,
   g.ksplit = 1;
   return g;
 }
+
+/* Blocked right-looking LU with partial pivoting of a batch of matrices
+ * (getrf semantics): per panel of LU_NB columns, the panel factorization with
+ * whole-row swaps, the triangular solve for U12 and the trailing update
+ * A22 -= L21 U12 as one grouped GEMM over the batch. */
+void Engine::lu_batch(std::vector<LuDesc>& descs) {
+  if (descs.empty()) return;
+  int maxn = 0;
+  for (auto& d : descs) maxn = std::max(maxn, d.n);
+  const LuDesc* dd = arena_.push(descs, stream_);
+  for (int p0 = 0; p0 < maxn; p0 += LU_NB) {
+    launch_lu_panel(dd, (int)descs.size(), p0, LU_NB, stream_);
+    std::vector<TrsmDesc> tr;
+    std::vector<GemmDesc> g;
+    int nwg = 0;
+    for (auto& d : descs) {
+      const int pw = std::min(LU_NB, d.n - p0);
+      const int rest = d.n - p0 - pw;
+      if (pw <= 0 || rest <= 0) continue;
+      TrsmDesc t{};
+      t.T = d.A + p0 + (size_t)p0 * d.ld;
+      t.B = d.A + p0 + (size_t)(p0 + pw) * d.ld;
+      t.ldt = d.ld;
+      t.ldb = d.ld;
+      t.pw = pw;
+      t.upper = 0;
+      t.ncols = rest;
+      t.chunk0 = nwg;
+      nwg += (rest + 63) / 64;
+      tr.push_back(t);
+      g.push_back(gemm_desc(rest, rest, pw, 0, d.A + (p0 + pw) + (size_t)p0 * d.ld, d.ld,
+                            d.A + p0 + (size_t)(p0 + pw) * d.ld, d.ld,
+                            d.A + (p0 + pw) + (size_t)(p0 + pw) * d.ld, d.ld, -1.0, 1.0));
+    }
+    if (!tr.empty()) {
+      launch_trsm(arena_.push(tr, stream_), (int)tr.size(), nwg, stream_);
+      run_gemm(g);
+    }
+  }
+}
+
+/* B <- A^-1 B for a batch of LU-factored matrices: row interchanges, then
+ * blocked forward (unit L) and backward (U) substitution, each block a small
+ * triangular solve followed by a grouped GEMM update of the remaining rows. */
+void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
+  if (descs.empty()) return;
+  int maxn = 0;
+  {
+    std::vector<PermDesc> pd;
+    int nwg = 0;
+    for (auto& d : descs) {
+      maxn = std::max(maxn, d.n);
+      PermDesc p{};
+      p.B = d.B;
+      p.piv = d.piv;
+      p.ldb = d.ldb;
+      p.n = d.n;
+      p.ncols = d.ncols;
+      p.chunk0 = nwg;
+      nwg += (d.ncols + 255) / 256;
+      pd.push_back(p);
+    }
+    launch_permute(arena_.push(pd, stream_), (int)pd.size(), nwg, stream_);
+  }
+  for (int dir = 0; dir < 2; dir++) {
+    const int nblk = (maxn + LU_NB - 1) / LU_NB;
+    for (int bi = 0; bi < nblk; bi++) {
+      std::vector<TrsmDesc> tr;
+      std::vector<GemmDesc> g;
+      int nwg = 0;
+      for (auto& d : descs) {
+        const int nb_d = (d.n + LU_NB - 1) / LU_NB;
+        const int blk = dir == 0 ? bi : nb_d - 1 - bi;  // backward: from the last block
+        if (blk < 0 || blk >= nb_d) continue;
+        const int p0 = blk * LU_NB;
+        const int pw = std::min(LU_NB, d.n - p0);
+        TrsmDesc t{};
+        t.T = d.LU + p0 + (size_t)p0 * d.ld_lu;
+        t.B = d.B + p0;
+        t.ldt = d.ld_lu;
+        t.ldb = d.ldb;
+        t.pw = pw;
+        t.upper = dir;
+        t.ncols = d.ncols;
+        t.chunk0 = nwg;
+        nwg += (d.ncols + 63) / 64;
+        tr.push_back(t);
+        if (dir == 0 && p0 + pw < d.n)  // rows below: B2 -= L21 B1
+          g.push_back(gemm_desc(d.n - p0 - pw, d.ncols, pw, 0, d.LU + (p0 + pw) + (size_t)p0 * d.ld_lu,
+                                d.ld_lu, d.B + p0, d.ldb, d.B + p0 + pw, d.ldb, -1.0, 1.0));
+        if (dir == 1 && p0 > 0)  // rows above: B0 -= U01 B1
+          g.push_back(gemm_desc(p0, d.ncols, pw, 0, d.LU + (size_t)p0 * d.ld_lu, d.ld_lu, d.B + p0,
+                                d.ldb, d.B, d.ldb, -1.0, 1.0));
+      }
+      launch_trsm(arena_.push(tr, stream_), (int)tr.size(), nwg, stream_);
+      run_gemm(g);
+    }
+  }
+}
+
 
 /* ------------------------------------------------------------------ */
 /* compression: storeLRinTree (HODLR_Matrix.cpp:257-430)               */
@@ -433,7 +531,7 @@ void Engine::aca_compress() {
     m_fin.clear();
     m_chunk.clear();
     for (int b = 0; b < nb; b++) {
-      nchunk[b] = std::max(1, (cap[blk[b].level] + 255) / 256);
+      nchunk[b] = std::max(1, (cap[blk[b].level] + 63) / 64);
       if (nchunk[b] > MAX_FIN_CHUNKS) throw std::runtime_error("ACA rank capacity exceeds 16384");
     }
     for (int b : active) {
@@ -577,9 +675,9 @@ void Engine::factor_leaves() {
     sd.push_back(s);
     maxs = std::max(maxs, lf.s);
   }
-  launch_lu(arena_.push(ld, stream_), (int)ld.size(), stream_);
+  lu_batch(ld);
   launch_set_identity(arena_.push(id, stream_), (int)id.size(), stream_);
-  lu_solve_batch(sd, maxs);
+  lu_solve_batch(sd);
   leaves_factored_ = true;
 }
 
@@ -621,7 +719,7 @@ void Engine::factorize(int which) {
     if (offd > 0 && !level_leaves_[d].empty()) {
       size_t need = 0;
       for (int li : level_leaves_[d]) need += (size_t)leaves_[li].s * offd;
-      if (tmp_.n < need) tmp_.alloc(need);
+      grow(tmp_, need);
       std::vector<GemmDesc> g;
       std::vector<CopyDesc> cp;
       size_t to = 0;
@@ -644,8 +742,8 @@ void Engine::factorize(int which) {
       zneed += (size_t)K * ncols;
       yneed += (size_t)K * std::max(offd, 1);
     }
-    if (tmp2_.n < zneed) tmp2_.alloc(zneed);
-    if (tmp_.n < yneed) tmp_.alloc(yneed);
+    grow(tmp2_, zneed);
+    grow(tmp_, yneed);
     std::vector<GemmDesc> gz, gu;
     std::vector<CapDesc> cd;
     std::vector<LuDesc> lu;
@@ -699,9 +797,9 @@ void Engine::factorize(int which) {
     }
     run_gemm(gz);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
-    launch_lu(arena_.push(lu, stream_), (int)lu.size(), stream_);
+    lu_batch(lu);
     if (offd > 0) {
-      lu_solve_batch(sd, maxK);
+      lu_solve_batch(sd);
       run_gemm(gu);
     }
   }
@@ -725,13 +823,13 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
   const int ld = ntot_;
   const size_t N = (size_t)ntot_;
-  if (xw_.n < N * nrhs) xw_.alloc(N * nrhs);
+  grow(xw_, N * nrhs);
   double* X = xw_.p;
   HODLR_HIP_CHECK(hipMemcpy2DAsync(X, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
                                    N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
   // leaves: X[rows] <- A_leaf^-1 X[rows]
   {
-    if (tmp_.n < N * nrhs) tmp_.alloc(N * nrhs);
+    grow(tmp_, N * nrhs);
     std::vector<GemmDesc> g;
     std::vector<CopyDesc> cp;
     size_t to = 0;
@@ -749,8 +847,8 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
     const int offd = off_[d];
     size_t zneed = 0;
     for (int id : level_nodes_[d]) zneed += (size_t)(nodes_[id].kt + nodes_[id].kb) * nrhs;
-    if (tmp2_.n < zneed) tmp2_.alloc(zneed);
-    if (tmp_.n < zneed) tmp_.alloc(zneed);
+    grow(tmp2_, zneed);
+    grow(tmp_, zneed);
     std::vector<GemmDesc> gz, gu;
     std::vector<CapDesc> cd;
     std::vector<SolveDesc> sd;
@@ -797,7 +895,7 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
     }
     run_gemm(gz);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
-    lu_solve_batch(sd, maxK);
+    lu_solve_batch(sd);
     run_gemm(gu);
   }
   HODLR_HIP_CHECK(hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), X, N * sizeof(double),
@@ -867,7 +965,7 @@ void Engine::matvec_device(const double* dX, int ldx, int nrhs, double* dY, int 
     if (level_nodes_[d].empty()) continue;
     size_t need = 0;
     for (int id : level_nodes_[d]) need += (size_t)(nodes_[id].kt + nodes_[id].kb) * nrhs;
-    if (tmp2_.n < need) tmp2_.alloc(need);
+    grow(tmp2_, need);
     std::vector<GemmDesc> g1, g2;
     size_t o = 0;
     for (int id : level_nodes_[d]) {

@@ -32,6 +32,8 @@
 
 namespace hodlr {
 
+constexpr int LU_NB = 64;  // panel width of the blocked LU and triangular solves
+
 template <class T>
 struct DBuf {
   T* p = nullptr;
@@ -170,7 +172,9 @@ class Engine {
   void factor_leaves();
   void run_gemm(std::vector<GemmDesc>& descs);
   void ensure_ws(size_t doubles);
-  void lu_solve_batch(std::vector<SolveDesc>& descs, int max_n);
+  void grow(DBuf<double>& b, size_t doubles);
+  void lu_batch(std::vector<LuDesc>& descs);
+  void lu_solve_batch(std::vector<SolveDesc>& descs);
 
   std::vector<InstanceSpec> specs_;
   int ntot_ = 0, nlev_ = 0;
