@@ -1,17 +1,28 @@
 """Benchmark of the HODLR hot path on one MI355X.
 
-One step = compression (ACA, fused kernel assembly) + recSM factorization +
-solve of the 2D Gaussian workload: N = 262144 points uniform in the unit
-square in kd order, K(x,y) = exp(-|x-y|^2 / 0.05^2) + 1e-4 on the diagonal,
-leaf 256, ACA tolerance 1e-10, 64 right-hand sides ~ N(0,1).  Inputs (points
-and right-hand sides) are resident in device memory before the timed region.
+Workload (the N = 262144 two-dimensional Gaussian system): points uniform in
+the unit square in kd order, K(x,y) = exp(-|x-y|^2 / 0.05^2) + 1e-4 on the
+diagonal, leaf 256, ACA tolerance 1e-10, 64 right-hand sides ~ N(0,1).
+Points and right-hand sides are resident in device memory before the timed
+region.
 
-Prints one JSON line (the contract documented in the repository), with the
-roofline of the dominant kernel (ACA) and a CPU baseline measured on a
-bounded sample by the oracle.
+One step = one hodlr_solve_device call on a fresh handle: compression (ACA
+with fused kernel assembly), recSM factorization, solve, and flexible-GMRES
+refinement against the FP64 HODLR operator until every column reaches a
+relative residual of 1e-10.  The preconditioner of the timed path is the FP64
+recSM factorization: the plain recSM solution alone stops near 1e-7 on this
+system (condition number ~1e7).  The mixed-precision factorization (FP32
+node products) is run once beside it and reported with its outcome.
+
+Prints one JSON line: the contract fields, the roofline of the dominant
+kernels (the ACA row and column residual passes, HBM bound, timed with HIP
+events on the engine's stream), and a CPU baseline measured by the oracle on
+a bounded sample.
 """
 import argparse
+import ctypes as C
 import json
+import math
 import os
 import sys
 import time
@@ -23,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor+solve time at N=256K to 1e-10 rel. residual: mixed+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "aca_gauss2d_262144_traffic.txt")
 
 
 def parse():
@@ -32,9 +44,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=1)
     p.add_argument("--n", type=int, default=262144)
     p.add_argument("--nrhs", type=int, default=64)
-    p.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
+    p.add_argument("--target", type=float, default=1e-10)
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--cpu-n", type=int, default=8192)
+    p.add_argument("--no-mixed", action="store_true")
+    p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
     return p.parse_args()
 
 
@@ -43,38 +57,62 @@ def workload(n, nrhs, seed=12345):
     return I.points_2d_kd(n, seed), I.rhs_normal(n, nrhs, seed + 1)
 
 
-def gpu_step(pts, dB, dX, n, nrhs, precision):
-    import ctypes as C
-
-    from dense_hodlr_amd import HODLR_Matrix, _lib
-
+def make_handle(pts, target, precision, max_steps):
+    from dense_hodlr_amd import HODLR_Matrix
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
     g.set_LRTolerance(1e-10)
-    if precision == "mixed":
-        g.set_precision("mixed")
+    g.set_precision(precision)
+    g.set_refinement(target, max_steps)
+    return g
+
+
+def solve_device(g, dB, dX, n, nrhs):
+    from dense_hodlr_amd import _lib
     t0 = time.perf_counter()
     _lib.check(_lib.lib().hodlr_solve_device(g._h, _lib.FACTOR_SM, C.c_void_p(dB), n, nrhs,
                                              C.c_void_p(dX), n), g._h)
-    wall = time.perf_counter() - t0
-    return g, wall
+    return time.perf_counter() - t0
 
 
-def cpu_baseline(n_sample, nrhs):
+def read_traffic():
+    """HBM bytes per ACA iteration (row + column launch) from the committed
+    rocprofv3 --pmc summary, or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    vals = {}
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                try:
+                    vals[k.strip()] = float(v.split()[0])
+                except (ValueError, IndexError):
+                    pass
+    return vals.get("hbm_bytes_per_iteration")
+
+
+def cpu_baseline(sizes, nrhs):
+    """The oracle's recSM (compress + factor + solve, one thread) at two
+    sample sizes of the same system; the larger is the reported sample, the
+    pair gives the exponent of a power-law extrapolation to the workload."""
     from oracle import pyoracle
-    pts, B = workload(n_sample, nrhs, seed=777)
-    o = pyoracle.OracleHODLR(n_sample, leaf=256, points=pts, family="gaussian",
-                             length_scale=0.05, diag=1e-4)
-    o.set_LRTolerance(1e-10)
-    t0 = time.perf_counter()
-    o.recSM_Solve(B)
-    t = time.perf_counter() - t0
-    return t, o.times()
+    res = []
+    for n in sizes:
+        pts, B = workload(n, nrhs, seed=777)
+        o = pyoracle.OracleHODLR(n, leaf=256, points=pts, family="gaussian",
+                                 length_scale=0.05, diag=1e-4)
+        o.set_LRTolerance(1e-10)
+        t0 = time.perf_counter()
+        o.recSM_Solve(B)
+        res.append((n, time.perf_counter() - t0, o.times()))
+    return res
 
 
 def main():
     a = parse()
     if a.gpus != 1:
-        print(json.dumps({"error": "this engine runs on one GPU; --gpus must be 1"}))
+        print(json.dumps({"error": "one GPU only: this build runs the whole system on a single "
+                                   "MI355X; --gpus must be 1"}))
         sys.exit(2)
     import torch
 
@@ -83,40 +121,104 @@ def main():
     dB = torch.from_numpy(np.ascontiguousarray(B.T)).to("cuda")  # column-major n x nrhs
     dX = torch.empty_like(dB)
     torch.cuda.synchronize()
+
+    def step(precision="fp64_refined", max_steps=30):
+        g = make_handle(pts, a.target, precision, max_steps)
+        wall = solve_device(g, dB.data_ptr(), dX.data_ptr(), n, nrhs)
+        return g, wall
+
     for _ in range(a.warmup):
-        gpu_step(pts, dB.data_ptr(), dX.data_ptr(), n, nrhs, a.precision)
-    torch.cuda.synchronize()
-    times, st = [], None
-    t_all = time.perf_counter()
-    for _ in range(a.steps):
-        g, wall = gpu_step(pts, dB.data_ptr(), dX.data_ptr(), n, nrhs, a.precision)
-        st = g.stats()
-        times.append(st["compress_time"] + st["factor_time"] + st["solve_time"])
+        g, _ = step()
         del g
     torch.cuda.synchronize()
+    walls, st, g = [], None, None
+    t_all = time.perf_counter()
+    for _ in range(a.steps):
+        g = None
+        g, wall = step()
+        st = g.stats()
+        walls.append(wall)
+    torch.cuda.synchronize()
     elapsed = time.perf_counter() - t_all
-    X = dX.cpu().numpy().T
     ms = 1e3 * elapsed / a.steps
-    value = float(np.median(times))
+    value = float(np.median(walls))
+
+    # independent check of the last solution: its residual against A_H, in
+    # double-double arithmetic and, for comparison, in plain FP64
+    X = dX.cpu().numpy().T.copy(order="F")
+    bn = np.linalg.norm(B, axis=0)
+    res = float(np.max(np.linalg.norm(g.residual(B, X), axis=0) / bn))
+    res64 = float(np.max(np.linalg.norm(B - g.matvec(X), axis=0) / bn))
+    g = None
 
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
-        "config": {"workload": f"gauss2d_kd N={n} l=0.05 diag=1e-4 leaf=256 tol=1e-10 nrhs={nrhs}",
-                   "precision": a.precision},
+        "config": {"workload": f"gauss2d_kd N={n} gaussian l=0.05 shift=1e-4 leaf=256 "
+                               f"aca_tol=1e-10 nrhs={nrhs} target={a.target:g}",
+                   "path": "plain FP64: ACA + recSM factor + solve + FGMRES refinement"},
         "phases_s": {"compress": st["compress_time"], "factor": st["factor_time"],
-                     "solve": st["solve_time"]},
+                     "solve_and_refine": st["solve_time"]},
+        "refine_steps": st["refine_iterations"],
+        "residual_reported": st["last_residual"],
+        "residual_checked": res,
+        "residual_fp64_evaluation": res64,
         "ranks_max_per_level": st["max_rank"],
         "aca_iterations": st["aca_iterations"],
     }
+
+    if not a.no_mixed:
+        from dense_hodlr_amd import HodlrError
+        g = make_handle(pts, a.target, "mixed", 20)
+        try:
+            wall = solve_device(g, dB.data_ptr(), dX.data_ptr(), n, nrhs)
+            sm = g.stats()
+            ok = sm["last_residual"] <= a.target
+            out["mixed"] = {"reached_target": bool(ok), "wall_s": wall,
+                            "residual": sm["last_residual"], "refine_steps": sm["refine_iterations"],
+                            "factor_s": sm["factor_time"],
+                            "speedup_vs_plain": (value / wall) if ok else None}
+        except HodlrError as e:
+            out["mixed"] = {"reached_target": False, "error": str(e)}
+        del g
+
+    if not a.no_roofline:
+        # one extra compression with HIP events around every ACA row/column
+        # launch (the timed steps above run without them)
+        g = make_handle(pts, a.target, "fp64_refined", 30)
+        g.set_profiling(True)
+        g.storeLRinTree()
+        sp = g.stats()
+        t_aca = sp["aca_row_time"] + sp["aca_col_time"]
+        iters = max(1, sp["aca_launches"])
+        achieved = sp["aca_bytes"] / t_aca / 1e9
+        traffic = read_traffic()
+        out["roofline"] = {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "kernel": "aca_row_kernel + aca_col_kernel (one ACA iteration over all active blocks)",
+            "algorithmic_bytes_per_launch": sp["aca_bytes"] / iters,
+            "avg_launch_us": {"row": 1e6 * sp["aca_row_time"] / iters,
+                              "col": 1e6 * sp["aca_col_time"] / iters},
+            "launches": iters,
+        }
+        del g
+
     if not a.no_cpu_baseline:
-        t, parts = cpu_baseline(a.cpu_n, nrhs)
-        out["cpu_baseline"] = {"value": t, "unit": "s", "cores": 1, "kind": "port",
-                               "sample": f"oracle recSM compress+factor+solve, same kernel/leaf/tol, "
-                                         f"N={a.cpu_n}, {nrhs} RHS, single thread",
-                               "phases_s": parts}
-    print(json.dumps(out))
+        pts_ = cpu_baseline(a.cpu_n, nrhs)
+        (n0, t0, _), (n1, t1, parts) = pts_[0], pts_[-1]
+        p = math.log(t1 / t0) / math.log(n1 / n0) if t1 > 0 and t0 > 0 and n1 != n0 else 2.0
+        out["cpu_baseline"] = {
+            "value": t1, "unit": "s", "cores": 1, "kind": "port",
+            "sample": f"oracle recSM compress+factor+solve (no refinement), same kernel, leaf, "
+                      f"tolerance and {nrhs} RHS at N={n1}, one thread",
+            "phases_s": parts,
+            "sizes_s": {str(k): v for k, v, _ in pts_},
+            "extrapolated_to_workload_s": t1 * (n / n1) ** p,
+            "extrapolation_exponent": p,
+        }
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":

@@ -23,16 +23,16 @@ ERRORS = {
 
 KERNELS = {"gaussian": 0, "matern32": 1, "se": 2, "imq": 3, "quadratic": 4, "exponential": 5}
 LR_METHODS = {"partialPiv_ACA": 0, "SVD": 1}
-PRECISIONS = {"fp64": 0, "mixed": 1}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp64_refined": 2}
 FACTOR_SM, FACTOR_LU = 0, 1
 
 # every symbol include/dense_hodlr.h declares
 EXPORTS = [
     "hodlr_create_kernel", "hodlr_create_kernel_callback", "hodlr_create_dense", "hodlr_destroy",
     "hodlr_set_lr_method", "hodlr_set_tolerance", "hodlr_set_min_pivot", "hodlr_set_precision",
-    "hodlr_set_refinement", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
+    "hodlr_set_refinement", "hodlr_set_profiling", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
     "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
-    "hodlr_matvec", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
+    "hodlr_matvec", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
     "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
 ]
 
@@ -63,6 +63,10 @@ class Stats(C.Structure):
         ("max_rank", C.c_int * 64),
         ("mean_rank", C.c_double * 64),
         ("device_bytes", C.c_longlong),
+        ("aca_row_time", C.c_double),
+        ("aca_col_time", C.c_double),
+        ("aca_launches", C.c_int),
+        ("aca_bytes", C.c_double),
     ]
 
 
@@ -99,6 +103,7 @@ def lib():
         "hodlr_set_min_pivot": (C.c_int, [h, C.c_double]),
         "hodlr_set_precision": (C.c_int, [h, C.c_int]),
         "hodlr_set_refinement": (C.c_int, [h, C.c_double, C.c_int]),
+        "hodlr_set_profiling": (C.c_int, [h, C.c_int]),
         "hodlr_compress": (C.c_int, [h]),
         "hodlr_factorize": (C.c_int, [h, C.c_int]),
         "hodlr_solve_sm": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
@@ -107,6 +112,7 @@ def lib():
         "hodlr_logabsdet": (C.c_int, [h, dp]),
         "hodlr_determinant": (C.c_int, [h, dp]),
         "hodlr_matvec": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_residual": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
         "hodlr_get_stats": (C.c_int, [h, C.POINTER(Stats)]),
         "hodlr_rows": (C.c_int, [h]),
         "hodlr_num_nodes": (C.c_int, [h]),

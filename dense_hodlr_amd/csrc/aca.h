@@ -40,8 +40,11 @@ struct AcaLaunch {
 
 void aca_launch_init(const AcaBlock* d_blocks, AcaState* d_state, uint32_t* d_bits, int nblocks,
                      hipStream_t s);
+/* One ACA iteration (row, column, finalize launches).  With ev non-null,
+ * events ev[0..2] are recorded before the row phase, between the row and
+ * column phases and after the column phase. */
 void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
-                          hipStream_t s);
+                          hipStream_t s, hipEvent_t* ev = nullptr);
 void aca_launch_post(const AcaLevels& L, const AcaBlock* d_blocks, AcaState* d_state,
                      const EntrySource* d_srcs, int* d_ranks, int nblocks, hipStream_t s);
 

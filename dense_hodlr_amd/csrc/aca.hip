@@ -597,19 +597,22 @@ static void launch_col(const AcaLevels& L, const AcaLaunch& a, const EntrySource
 }
 
 void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
-                          hipStream_t s) {
+                          hipStream_t s, hipEvent_t* ev) {
+  if (ev) HODLR_HIP_CHECK(hipEventRecord(ev[0], s));
   switch (a.jpt) {
     case 1: launch_row<1>(L, a, d_srcs, s); break;
     case 2: launch_row<2>(L, a, d_srcs, s); break;
     default: launch_row<4>(L, a, d_srcs, s);
   }
   HODLR_HIP_CHECK(hipGetLastError());
+  if (ev) HODLR_HIP_CHECK(hipEventRecord(ev[1], s));
   switch (a.ipt) {
     case 1: launch_col<1>(L, a, d_srcs, s); break;
     case 2: launch_col<2>(L, a, d_srcs, s); break;
     default: launch_col<4>(L, a, d_srcs, s);
   }
   HODLR_HIP_CHECK(hipGetLastError());
+  if (ev) HODLR_HIP_CHECK(hipEventRecord(ev[2], s));
   aca_fin_kernel<<<a.n_fin_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_fin, a.wg2chunk_fin, a.nchunk,
                                              a.state, a.bits, a.fin_part, a.counters, a.tol,
                                              a.min_rank, a.max_rank);

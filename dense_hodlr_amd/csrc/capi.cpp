@@ -174,6 +174,10 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
   return guard(h, [&] { h->eng->set_refinement(target, max_iters); });
 }
 
+int hodlr_set_profiling(hodlr_handle h, int on) {
+  return guard(h, [&] { h->eng->set_profiling(on != 0); });
+}
+
 int hodlr_compress(hodlr_handle h) {
   return guard(h, [&] { h->eng->compress(); });
 }
@@ -229,6 +233,27 @@ int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, 
   });
 }
 
+int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const double* X, int ldx,
+                   double* R, int ldr) {
+  return guard(h, [&] {
+    const int n = h->eng->rows();
+    if (!B || !X || !R || nrhs < 0 || ldb < n || ldx < n || ldr < n)
+      throw std::invalid_argument("bad arguments");
+    if (nrhs == 0) return;
+    hodlr::DBuf<double> db, dx, dr;
+    db.alloc((size_t)n * nrhs);
+    dx.alloc((size_t)n * nrhs);
+    dr.alloc((size_t)n * nrhs);
+    HODLR_HIP_CHECK(hipMemcpy2D(db.p, (size_t)n * 8, B, (size_t)ldb * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyHostToDevice));
+    HODLR_HIP_CHECK(hipMemcpy2D(dx.p, (size_t)n * 8, X, (size_t)ldx * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyHostToDevice));
+    h->eng->residual_dd(db.p, n, dx.p, n, nrhs, dr.p, n);
+    HODLR_HIP_CHECK(hipMemcpy2D(R, (size_t)ldr * 8, dr.p, (size_t)n * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyDeviceToHost));
+  });
+}
+
 int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
   return guard(h, [&] {
     if (!out) throw std::invalid_argument("null output");
@@ -255,6 +280,10 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     for (int l = 0; l < 64; l++)
       if (cnt[l]) out->mean_rank[l] /= cnt[l];
     out->device_bytes = e.device_bytes();
+    out->aca_row_time = s.aca_row_time;
+    out->aca_col_time = s.aca_col_time;
+    out->aca_launches = s.aca_launches;
+    out->aca_bytes = s.aca_bytes;
   });
 }
 

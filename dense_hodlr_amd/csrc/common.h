@@ -74,19 +74,23 @@ struct AcaState {
 };
 
 /* One problem of a grouped GEMM: C = alpha * op(A) * B + beta * C,
- * op(A) = A (trans_a 0, A is M x K) or A^T (trans_a 1, A is K x M). */
-struct GemmDesc {
+ * op(A) = A (trans_a 0, A is M x K) or A^T (trans_a 1, A is K x M).
+ * Element type T: double (FP64 path) or float (FP32 bulk of the mixed path). */
+template <class T>
+struct GemmDescT {
   int M, N, K;
   int trans_a;
   int lda, ldb, ldc;
-  int ksplit;     /* number of K slices (>=1) */
-  int tile0;      /* first tile index of this problem */
-  long long ws_off; /* workspace offset (doubles) for split-K partials */
-  double alpha, beta;
-  const double* A;
-  const double* B;
-  double* C;
+  int ksplit;       /* number of K slices (>=1) */
+  int tile0;        /* first tile index of this problem */
+  long long ws_off; /* workspace offset (elements) for split-K partials */
+  T alpha, beta;
+  const T* A;
+  const T* B;
+  T* C;
 };
+using GemmDesc = GemmDescT<double>;
+using GemmDescF = GemmDescT<float>;
 
 }  // namespace hodlr
 

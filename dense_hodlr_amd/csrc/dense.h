@@ -49,16 +49,44 @@ struct CopyDesc {
 };
 
 /* Assemble the small node matrix and its right-hand side from the Gram
- * products of one node (see engine.cpp for the layout). */
-struct CapDesc {
-  const double* Zt; /* kb x ncols, ld kb: V_b^T [X_top | U_t^SM] */
-  const double* Zb; /* kt x ncols, ld kt: V_t^T [X_bot | U_b^SM] */
+ * products of one node (see engine.cpp for the layout).  The products are
+ * FP64 (plain path) or FP32 (mixed path); C and Y are always FP64. */
+template <class TZ>
+struct CapDescT {
+  const TZ* Zt; /* kb x ncols, ld kb: V_b^T [X_top | U_t^SM] */
+  const TZ* Zb; /* kt x ncols, ld kt: V_t^T [X_bot | U_b^SM] */
   double* C;        /* K x K, K = kt + kb */
   double* Y;        /* K x nanc, ld K */
   int kt, kb;
   int nanc;         /* ancestor (right-hand side) columns */
   int own0;         /* column of U^SM own block inside Z */
   int lu_order;     /* 0: recSM capacitance; 1: recLU Schur ordering */
+  int pad_;
+};
+using CapDesc = CapDescT<double>;
+using CapDescF = CapDescT<float>;
+
+/* m x n block copy with conversion between FP64 and FP32. */
+template <class S, class D>
+struct ConvDesc {
+  const S* src;
+  D* dst;
+  int m, n;
+  int lds, ldd;
+};
+
+/* FP64 -> FP32 conversion of one low-rank block U V^T (U: mu x k rows of a
+ * level panel, V: mv x k) with every term rebalanced: u_c s_c and v_c / s_c,
+ * s_c = sqrt(|v_c| / |u_c|).  The product is unchanged; partial-pivot ACA
+ * leaves 1/pivot in u_c, which for the small late terms exceeds the FP32
+ * range while v_c is tiny. */
+struct BalanceDesc {
+  const double* U;
+  const double* V;
+  float* U32;
+  float* V32;
+  int mu, mv, k;
+  int ldu, ldv, ldu32, ldv32;
   int pad_;
 };
 
@@ -92,7 +120,32 @@ void launch_assemble(const AssembleDesc* d, int count, int max_elems, const Entr
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);
 void launch_copy(const CopyDesc* d, int count, hipStream_t s);
 void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod, hipStream_t s);
-void launch_capacitance(const CapDesc* d, int count, hipStream_t s);
+template <class TZ>
+void launch_capacitance(const CapDescT<TZ>* d, int count, hipStream_t s);
+template <class S, class D>
+void launch_convert(const ConvDesc<S, D>* d, int count, hipStream_t s);
+/* one workgroup per (term c < kmax, block); terms c >= k of a block are left
+ * untouched */
+void launch_balance(const BalanceDesc* d, int count, int kmax, hipStream_t s);
+/* out[c] = sum_i X[i + c*ld]^2 for c < ncols (one workgroup per column, fixed
+ * reduction order) */
+void launch_colnorm2(const double* X, int n, int ld, int ncols, double* out, hipStream_t s);
+/* X += alpha * D (FP32 correction) over an n x ncols block */
+void launch_add_f32(double* X, int ldx, const float* D, int ldd, int n, int ncols, hipStream_t s);
+/* Batched column operations of the block GMRES: dots of w with nvec basis
+ * blocks V + i*vstride (out is nvec x ncols), the combination
+ * w = (assign ? 0 : w) + sign * sum_i V_i coef(i,:), and column scaling. */
+void launch_multi_dot(const double* V, size_t vstride, int nvec, const double* w, int n, int ld,
+                      int ncols, double* out, hipStream_t s);
+void launch_multi_axpy(const double* V, size_t vstride, int nvec, const double* coef, double* w,
+                       int n, int ld, int ncols, double sign, int assign, hipStream_t s);
+void launch_col_scale(double* X, int ld, int n, int ncols, const double* sc, hipStream_t s);
+/* Adds the number of non-finite entries of x[0, n) to *out (diagnostics). */
+template <class T>
+void launch_nonfinite(const T* x, size_t n, unsigned long long* out, hipStream_t s);
+/* R = B - Y over an n x ncols block */
+void launch_sub(const double* B, int ldb, const double* Y, int ldy, double* R, int ldr, int n,
+                int ncols, hipStream_t s);
 
 }  // namespace hodlr
 

@@ -74,8 +74,9 @@ __global__ void __launch_bounds__(TPB)
  *                               Y = [V_t^T X_bot ; V_b^T X_top]
  *   recLU (HODLR_Matrix.cpp:663-673): S = [[I_kb, V_b^T U_t^SM], [V_t^T U_b^SM, I_kt]],
  *                               Y = [V_b^T X_top ; V_t^T X_bot]            */
-__global__ void __launch_bounds__(TPB) capacitance_kernel(const CapDesc* __restrict__ d) {
-  const CapDesc c = d[blockIdx.y];
+template <class TZ>
+__global__ void __launch_bounds__(TPB) capacitance_kernel(const CapDescT<TZ>* __restrict__ d) {
+  const CapDescT<TZ> c = d[blockIdx.y];
   const int K = c.kt + c.kb;
   const int ka = c.lu_order ? c.kb : c.kt;  // rows of the first block
   const size_t totC = c.C ? (size_t)K * K : 0;  // solve phase: right-hand side only
@@ -88,12 +89,12 @@ __global__ void __launch_bounds__(TPB) capacitance_kernel(const CapDesc* __restr
       v = (i == j) ? 1.0 : 0.0;
     } else if (i < ka) {  // upper-right block
       const int jj = j - ka;
-      v = c.lu_order ? c.Zt[(size_t)i + (size_t)(c.own0 + jj) * c.kb]
-                     : c.Zb[(size_t)i + (size_t)(c.own0 + jj) * c.kt];
+      v = c.lu_order ? (double)c.Zt[(size_t)i + (size_t)(c.own0 + jj) * c.kb]
+                     : (double)c.Zb[(size_t)i + (size_t)(c.own0 + jj) * c.kt];
     } else {  // lower-left block
       const int ii = i - ka;
-      v = c.lu_order ? c.Zb[(size_t)ii + (size_t)(c.own0 + j) * c.kt]
-                     : c.Zt[(size_t)ii + (size_t)(c.own0 + j) * c.kb];
+      v = c.lu_order ? (double)c.Zb[(size_t)ii + (size_t)(c.own0 + j) * c.kt]
+                     : (double)c.Zt[(size_t)ii + (size_t)(c.own0 + j) * c.kb];
     }
     c.C[e] = v;
   }
@@ -102,10 +103,11 @@ __global__ void __launch_bounds__(TPB) capacitance_kernel(const CapDesc* __restr
     const int i = (int)(e % K), j = (int)(e / K);
     double v;
     if (i < ka)
-      v = c.lu_order ? c.Zt[(size_t)i + (size_t)j * c.kb] : c.Zb[(size_t)i + (size_t)j * c.kt];
+      v = c.lu_order ? (double)c.Zt[(size_t)i + (size_t)j * c.kb]
+                     : (double)c.Zb[(size_t)i + (size_t)j * c.kt];
     else
-      v = c.lu_order ? c.Zb[(size_t)(i - ka) + (size_t)j * c.kt]
-                     : c.Zt[(size_t)(i - ka) + (size_t)j * c.kb];
+      v = c.lu_order ? (double)c.Zb[(size_t)(i - ka) + (size_t)j * c.kt]
+                     : (double)c.Zt[(size_t)(i - ka) + (size_t)j * c.kb];
     c.Y[e] = v;
   }
 }
@@ -249,6 +251,136 @@ __global__ void __launch_bounds__(TPB) permute_kernel(const PermDesc* __restrict
   }
 }
 
+template <class S, class D>
+__global__ void __launch_bounds__(TPB) convert_kernel(const ConvDesc<S, D>* __restrict__ d) {
+  const ConvDesc<S, D> c = d[blockIdx.y];
+  const size_t tot = (size_t)c.m * c.n;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const int i = (int)(e % c.m), j = (int)(e / c.m);
+    c.dst[(size_t)i + (size_t)j * c.ldd] = (D)c.src[(size_t)i + (size_t)j * c.lds];
+  }
+}
+
+/* Max-norms of u_c and v_c (no overflow for any finite input), then the
+ * rebalanced FP32 copies. */
+__global__ void __launch_bounds__(TPB) balance_kernel(const BalanceDesc* __restrict__ d) {
+  const BalanceDesc b = d[blockIdx.y];
+  const int c = blockIdx.x;
+  if (c >= b.k) return;
+  const double* u = b.U + (size_t)c * b.ldu;
+  const double* v = b.V + (size_t)c * b.ldv;
+  double mu = 0.0, mv = 0.0;
+  for (int i = threadIdx.x; i < b.mu; i += TPB) mu = fmax(mu, fabs(u[i]));
+  for (int i = threadIdx.x; i < b.mv; i += TPB) mv = fmax(mv, fabs(v[i]));
+  __shared__ double su[TPB], sv[TPB];
+  su[threadIdx.x] = mu;
+  sv[threadIdx.x] = mv;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) {
+      su[threadIdx.x] = fmax(su[threadIdx.x], su[threadIdx.x + w]);
+      sv[threadIdx.x] = fmax(sv[threadIdx.x], sv[threadIdx.x + w]);
+    }
+    __syncthreads();
+  }
+  mu = su[0];
+  mv = sv[0];
+  const double s = (mu > 0.0 && mv > 0.0) ? sqrt(mv / mu) : 1.0;
+  const double is = 1.0 / s;
+  float* u32 = b.U32 + (size_t)c * b.ldu32;
+  float* v32 = b.V32 + (size_t)c * b.ldv32;
+  for (int i = threadIdx.x; i < b.mu; i += TPB) u32[i] = (float)(u[i] * s);
+  for (int i = threadIdx.x; i < b.mv; i += TPB) v32[i] = (float)(v[i] * is);
+}
+
+__global__ void __launch_bounds__(TPB)
+    colnorm2_kernel(const double* __restrict__ X, int n, int ld, double* __restrict__ out) {
+  __shared__ double sh[TPB];
+  const double* x = X + (size_t)blockIdx.x * ld;
+  double s = 0.0;
+  for (int i = threadIdx.x; i < n; i += TPB) s += x[i] * x[i];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(TPB) add_f32_kernel(double* __restrict__ X, int ldx,
+                                                      const float* __restrict__ D, int ldd, int n,
+                                                      int ncols) {
+  const size_t tot = (size_t)n * ncols;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    X[(size_t)i + (size_t)j * ldx] += (double)D[(size_t)i + (size_t)j * ldd];
+  }
+}
+
+__global__ void __launch_bounds__(TPB) sub_kernel(const double* __restrict__ B, int ldb,
+                                                  const double* __restrict__ Y, int ldy,
+                                                  double* __restrict__ R, int ldr, int n, int ncols) {
+  const size_t tot = (size_t)n * ncols;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const int i = (int)(e % n), j = (int)(e / n);
+    R[(size_t)i + (size_t)j * ldr] = B[(size_t)i + (size_t)j * ldb] - Y[(size_t)i + (size_t)j * ldy];
+  }
+}
+
+/* out[i + c*nvec] = V_i(:,c) . w(:,c) for basis vectors V_i = V + i*vstride. */
+__global__ void __launch_bounds__(TPB)
+    multi_dot_kernel(const double* __restrict__ V, size_t vstride, int nvec,
+                     const double* __restrict__ w, int n, int ld, double* __restrict__ out) {
+  __shared__ double sh[TPB];
+  const int c = blockIdx.x, i = blockIdx.y;
+  const double* v = V + (size_t)i * vstride + (size_t)c * ld;
+  const double* x = w + (size_t)c * ld;
+  double s = 0.0;
+  for (int r = threadIdx.x; r < n; r += TPB) s += v[r] * x[r];
+  sh[threadIdx.x] = s;
+  __syncthreads();
+  for (int h = TPB / 2; h > 0; h >>= 1) {
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[i + (size_t)c * nvec] = sh[0];
+}
+
+/* w(:,c) = (assign ? 0 : w(:,c)) + sign * sum_i V_i(:,c) coef(i,c) */
+__global__ void __launch_bounds__(TPB)
+    multi_axpy_kernel(const double* __restrict__ V, size_t vstride, int nvec,
+                      const double* __restrict__ coef, double* __restrict__ w, int n, int ld,
+                      int ncols, double sign, int assign) {
+  const size_t tot = (size_t)n * ncols;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    double s = 0.0;
+    for (int i = 0; i < nvec; i++)
+      s += V[(size_t)i * vstride + r + (size_t)c * ld] * coef[i + (size_t)c * nvec];
+    double* o = w + r + (size_t)c * ld;
+    *o = (assign ? 0.0 : *o) + sign * s;
+  }
+}
+
+__global__ void __launch_bounds__(TPB) col_scale_kernel(double* __restrict__ X, int ld, int n,
+                                                        int ncols, const double* __restrict__ s) {
+  const size_t tot = (size_t)n * ncols;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const int r = (int)(e % n), c = (int)(e / n);
+    X[r + (size_t)c * ld] *= s[c];
+  }
+}
+
+template <class T>
+__global__ void __launch_bounds__(TPB) nonfinite_kernel(const T* __restrict__ x, size_t n,
+                                                        unsigned long long* out) {
+  unsigned long long c = 0;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < n; e += (size_t)gridDim.x * TPB)
+    if (!isfinite((double)x[e])) c++;
+  if (c) atomicAdd(out, c);
+}
+
 int grid_x(size_t elems, int ndesc) {
   size_t g = (elems + TPB - 1) / TPB;
   size_t cap = ndesc > 64 ? 16 : 1024;
@@ -305,10 +437,75 @@ void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-void launch_capacitance(const CapDesc* d, int count, hipStream_t s) {
+template <class TZ>
+void launch_capacitance(const CapDescT<TZ>* d, int count, hipStream_t s) {
   if (count == 0) return;
   dim3 g(16, count);
-  capacitance_kernel<<<g, TPB, 0, s>>>(d);
+  capacitance_kernel<TZ><<<g, TPB, 0, s>>>(d);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+template void launch_capacitance<double>(const CapDesc*, int, hipStream_t);
+template void launch_capacitance<float>(const CapDescF*, int, hipStream_t);
+
+template <class S, class D>
+void launch_convert(const ConvDesc<S, D>* d, int count, hipStream_t s) {
+  if (count == 0) return;
+  dim3 g(count > 64 ? 16 : 512, count);
+  convert_kernel<S, D><<<g, TPB, 0, s>>>(d);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+template void launch_convert<double, float>(const ConvDesc<double, float>*, int, hipStream_t);
+template void launch_convert<float, double>(const ConvDesc<float, double>*, int, hipStream_t);
+template void launch_convert<float, float>(const ConvDesc<float, float>*, int, hipStream_t);
+
+void launch_balance(const BalanceDesc* d, int count, int kmax, hipStream_t s) {
+  if (count == 0 || kmax == 0) return;
+  dim3 g(kmax, count);
+  balance_kernel<<<g, TPB, 0, s>>>(d);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+template <class T>
+void launch_nonfinite(const T* x, size_t n, unsigned long long* out, hipStream_t s) {
+  nonfinite_kernel<T><<<1024, TPB, 0, s>>>(x, n, out);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+template void launch_nonfinite<float>(const float*, size_t, unsigned long long*, hipStream_t);
+template void launch_nonfinite<double>(const double*, size_t, unsigned long long*, hipStream_t);
+
+void launch_colnorm2(const double* X, int n, int ld, int ncols, double* out, hipStream_t s) {
+  if (ncols <= 0) return;
+  colnorm2_kernel<<<ncols, TPB, 0, s>>>(X, n, ld, out);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_add_f32(double* X, int ldx, const float* D, int ldd, int n, int ncols, hipStream_t s) {
+  add_f32_kernel<<<2048, TPB, 0, s>>>(X, ldx, D, ldd, n, ncols);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_multi_dot(const double* V, size_t vstride, int nvec, const double* w, int n, int ld,
+                      int ncols, double* out, hipStream_t s) {
+  if (nvec <= 0 || ncols <= 0) return;
+  dim3 g(ncols, nvec);
+  multi_dot_kernel<<<g, TPB, 0, s>>>(V, vstride, nvec, w, n, ld, out);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_multi_axpy(const double* V, size_t vstride, int nvec, const double* coef, double* w,
+                       int n, int ld, int ncols, double sign, int assign, hipStream_t s) {
+  multi_axpy_kernel<<<2048, TPB, 0, s>>>(V, vstride, nvec, coef, w, n, ld, ncols, sign, assign);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_col_scale(double* X, int ld, int n, int ncols, const double* sc, hipStream_t s) {
+  col_scale_kernel<<<2048, TPB, 0, s>>>(X, ld, n, ncols, sc);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_sub(const double* B, int ldb, const double* Y, int ldy, double* R, int ldr, int n,
+                int ncols, hipStream_t s) {
+  sub_kernel<<<2048, TPB, 0, s>>>(B, ldb, Y, ldy, R, ldr, n, ncols);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

@@ -6,6 +6,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdlib>
+#include <cstdio>
 #include <cstring>
 
 namespace hodlr {
@@ -48,6 +49,7 @@ size_t DescArena::bytes() const {
 /* construction                                                        */
 /* ------------------------------------------------------------------ */
 Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
+  trace_ = std::getenv("HODLR_TRACE") != nullptr;
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreate(&ev0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
@@ -57,6 +59,7 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
 
 Engine::~Engine() {
   if (stream_) (void)hipStreamSynchronize(stream_);
+  for (auto e : prof_ev_) (void)hipEventDestroy(e);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (stream_) (void)hipStreamDestroy(stream_);
@@ -187,6 +190,7 @@ void Engine::reset_attributes() {
     f.C.release();
     f.piv.release();
   }
+  for (auto& f : mfac_) f = MixedFactorization();
   U_.clear();
   V_.clear();
   stats_ = Stats();
@@ -217,7 +221,7 @@ void Engine::set_min_pivot(double p) {
 }
 
 void Engine::set_precision(int p) {
-  if (p != 0 && p != 1) throw std::invalid_argument("unknown precision");
+  if (p < 0 || p > 2) throw std::invalid_argument("unknown precision");
   if (p != precision_) {
     precision_ = p;
     for (auto& f : fac_) {
@@ -226,6 +230,7 @@ void Engine::set_precision(int p) {
       f.C.release();
       f.piv.release();
     }
+    for (auto& f : mfac_) f = MixedFactorization();
   }
 }
 
@@ -242,6 +247,13 @@ long long Engine::device_bytes() const {
   b += Lorig_.bytes() + Lfac_.bytes() + Linv_.bytes() + Lpiv_.bytes();
   for (auto& f : fac_) b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
   b += ws_.bytes() + tmp_.bytes() + tmp2_.bytes() + xw_.bytes();
+  for (auto& f : mfac_) {
+    b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
+    for (auto& x : f.V) b += x.bytes();
+  }
+  b += t32a_.bytes() + t32b_.bytes() + x32_.bytes();
+  b += rbuf_.bytes() + ybuf_.bytes() + zbuf_.bytes() + nbuf_.bytes() + basis_.bytes() +
+       zbasis_.bytes() + ddy_.bytes() + ddpart_.bytes() + ddt_.bytes();
   for (auto& x : d_coords_) b += x.bytes();
   for (auto& x : d_dense_) b += x.bytes();
   b += arena_.bytes();
@@ -265,21 +277,26 @@ void Engine::grow(DBuf<double>& b, size_t doubles) {
   }
 }
 
-void Engine::run_gemm(std::vector<GemmDesc>& descs) {
+template <class T>
+void Engine::run_gemm_t(std::vector<GemmDescT<T>>& descs) {
   if (descs.empty()) return;
   std::vector<int> split_ids;
   int ntiles = 0;
   const int bm = gemm_tile_m(descs);
   const size_t ws = gemm_plan(descs, split_ids, ntiles, 2048, bm);
-  if (ws) ensure_ws(ws);
-  const GemmDesc* d = arena_.push(descs, stream_);
+  if (ws) ensure_ws((ws * sizeof(T) + 7) / 8);
+  const GemmDescT<T>* d = arena_.push(descs, stream_);
   const int* ids = split_ids.empty() ? nullptr : arena_.push(split_ids, stream_);
-  launch_gemm(d, (int)descs.size(), ntiles, ws_.p, ids, (int)split_ids.size(), bm, stream_);
+  launch_gemm<T>(d, (int)descs.size(), ntiles, reinterpret_cast<T*>(ws_.p), ids,
+                 (int)split_ids.size(), bm, stream_);
 }
 
-static GemmDesc gemm_desc(int M, int N, int K, int ta, const double* A, int lda, const double* B,
-                          int ldb, double* C, int ldc, double alpha, double beta) {
-  GemmDesc g{};
+void Engine::run_gemm(std::vector<GemmDesc>& descs) { run_gemm_t<double>(descs); }
+
+template <class T>
+static GemmDescT<T> gemm_desc(int M, int N, int K, int ta, const T* A, int lda, const T* B, int ldb,
+                              T* C, int ldc, T alpha, T beta) {
+  GemmDescT<T> g{};
   g.M = M;
   g.N = N;
   g.K = K;
@@ -579,10 +596,30 @@ void Engine::aca_compress() {
   while (true) {
     if (launched > max_dim + 64)
       throw std::runtime_error("ACA did not terminate within m + n iterations");
-    for (int c = 0; c < check; c++) aca_launch_iteration(L, a, d_srcs_.p, stream_);
+    if (profiling_) {
+      while ((int)prof_ev_.size() < 3 * check) {
+        hipEvent_t e;
+        HODLR_HIP_CHECK(hipEventCreate(&e));
+        prof_ev_.push_back(e);
+      }
+      for (int c = 0; c < check; c++)
+        aca_launch_iteration(L, a, d_srcs_.p, stream_, prof_ev_.data() + 3 * c);
+    } else {
+      for (int c = 0; c < check; c++) aca_launch_iteration(L, a, d_srcs_.p, stream_);
+    }
     launched += check;
     HODLR_HIP_CHECK(hipMemcpyAsync(hs.data(), d_state.p, nb * sizeof(AcaState), hipMemcpyDeviceToHost, stream_));
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (profiling_) {
+      for (int c = 0; c < check; c++) {
+        float r = 0, q = 0;
+        HODLR_HIP_CHECK(hipEventElapsedTime(&r, prof_ev_[3 * c], prof_ev_[3 * c + 1]));
+        HODLR_HIP_CHECK(hipEventElapsedTime(&q, prof_ev_[3 * c + 1], prof_ev_[3 * c + 2]));
+        stats_.aca_row_time += r * 1e-3;
+        stats_.aca_col_time += q * 1e-3;
+        stats_.aca_launches += 1;
+      }
+    }
     std::vector<int> na;
     std::vector<char> grow(nl, 0);
     bool paused = false;
@@ -635,6 +672,13 @@ void Engine::aca_compress() {
   off_[0] = 0;
   for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
   ktot_ = off_[nlev_];
+  // algorithmic bytes of the residual passes: 8 (m+n) k(k+1)/2 per block
+  double bytes = 0;
+  for (int b = 0; b < nb; b++) {
+    const double k = hs[b].k;
+    bytes += 8.0 * (double)(blk[b].m + blk[b].n) * k * (k + 1) / 2;
+  }
+  stats_.aca_bytes = bytes;
 }
 
 void Engine::assemble_leaves() {
@@ -687,6 +731,14 @@ void Engine::factor_leaves() {
  * the ancestors' U^SM columns hold. */
 void Engine::factorize(int which) {
   if (which != 0 && which != 1) throw std::invalid_argument("unknown factorization");
+  if (precision_ == 1) {
+    factorize_mixed(which);
+    return;
+  }
+  factorize_fp64(which);
+}
+
+void Engine::factorize_fp64(int which) {
   compress();
   Factorization& F = fac_[which];
   if (F.done) return;
@@ -818,15 +870,35 @@ void Engine::factorize(int which) {
 void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double* dX, int ldx) {
   if (nrhs <= 0) return;
   if (ldb < ntot_ || ldx < ntot_) throw std::invalid_argument("leading dimension < n");
-  factorize(which);
-  Factorization& F = fac_[which];
+  if (precision_ != 0) {
+    solve_gmres(which, precision_ == 1, dB, ldb, nrhs, dX, ldx);
+    return;
+  }
+  factorize_fp64(which);
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
-  const int ld = ntot_;
   const size_t N = (size_t)ntot_;
   grow(xw_, N * nrhs);
-  double* X = xw_.p;
-  HODLR_HIP_CHECK(hipMemcpy2DAsync(X, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
+  HODLR_HIP_CHECK(hipMemcpy2DAsync(xw_.p, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
                                    N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
+  solve_fp64_inplace(which, xw_.p, nrhs);
+  HODLR_HIP_CHECK(hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), xw_.p, N * sizeof(double),
+                                   N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
+  HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
+  HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
+  float ms = 0;
+  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  stats_.solve_time = ms * 1e-3;
+  arena_.reset();
+}
+
+/* X <- A_H^-1 X with the FP64 recSM (recLU) factorization, X is ntot x nrhs
+ * with leading dimension ntot: all leaves, then the nodes level by level from
+ * the deepest (the post-order of HODLR_Matrix.cpp:594-610 regrouped by level;
+ * nodes of one level own disjoint rows). */
+void Engine::solve_fp64_inplace(int which, double* X, int nrhs) {
+  Factorization& F = fac_[which];
+  const int ld = ntot_;
+  const size_t N = (size_t)ntot_;
   // leaves: X[rows] <- A_leaf^-1 X[rows]
   {
     grow(tmp_, N * nrhs);
@@ -898,14 +970,6 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
     lu_solve_batch(sd);
     run_gemm(gu);
   }
-  HODLR_HIP_CHECK(hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), X, N * sizeof(double),
-                                   N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
-  HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
-  HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
-  float ms = 0;
-  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  stats_.solve_time = ms * 1e-3;
-  arena_.reset();
 }
 
 void Engine::solve_host(int which, const double* B, int ldb, int nrhs, double* X, int ldx) {
@@ -920,6 +984,472 @@ void Engine::solve_host(int which, const double* B, int ldb, int nrhs, double* X
   solve_device(which, dB.p, (int)N, nrhs, dX.p, (int)N);
   HODLR_HIP_CHECK(hipMemcpy2D(X, (size_t)ldx * sizeof(double), dX.p, N * sizeof(double),
                               N * sizeof(double), nrhs, hipMemcpyDeviceToHost));
+}
+
+/* ------------------------------------------------------------------ */
+/* mixed precision: FP32 bulk, FP64 leaf LU / node matrices / residuals */
+/* ------------------------------------------------------------------ */
+template <class T>
+void Engine::grow_t(DBuf<T>& b, size_t n) {
+  if (b.n < n) {
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    b.alloc(n);
+  }
+}
+
+template <class T>
+long long Engine::count_nonfinite(const T* x, size_t n) {
+  DBuf<unsigned long long> c;
+  c.alloc(1);
+  HODLR_HIP_CHECK(hipMemsetAsync(c.p, 0, sizeof(unsigned long long), stream_));
+  launch_nonfinite<T>(x, n, c.p, stream_);
+  unsigned long long h = 0;
+  HODLR_HIP_CHECK(hipMemcpyAsync(&h, c.p, sizeof(h), hipMemcpyDeviceToHost, stream_));
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  return (long long)h;
+}
+
+/* The recSM (or recLU) factorization of factorize(), with the U^SM panel and
+ * the V panels in FP32 and every bulk product an FP32 GEMM; the leaves' LU and
+ * inverses and the node matrices (built, factored and solved) stay FP64. */
+void Engine::factorize_mixed(int which) {
+  compress();
+  MixedFactorization& M = mfac_[which];
+  if (M.done) return;
+  HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
+  factor_leaves();
+  const size_t N = (size_t)ntot_;
+  const int ld = ntot_;
+  M.V.clear();
+  M.V.resize(nlev_);
+  M.W.alloc(std::max<size_t>(1, N * ktot_));
+  HODLR_HIP_CHECK(hipMemsetAsync(M.W.p, 0, M.W.bytes(), stream_));
+  {
+    // rebalanced FP32 copies of every node's two blocks; rows and terms
+    // outside them stay zero
+    std::vector<BalanceDesc> bd;
+    int kmax = 0;
+    for (int d = 0; d < nlev_; d++) {
+      if (K_[d] == 0) continue;
+      M.V[d].alloc(N * K_[d]);
+      HODLR_HIP_CHECK(hipMemsetAsync(M.V[d].p, 0, M.V[d].bytes(), stream_));
+      kmax = std::max(kmax, K_[d]);
+      float* W32 = M.W.p + N * off_[d];
+      for (int id : level_nodes_[d]) {
+        const TreeNodeH& nd = nodes_[id];
+        const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+        const int top = nd.row0, bot = nd.row0 + st;
+        // top block: U rows [top, top+st), V rows [bot, bot+sb); bottom the reverse
+        bd.push_back({U_[d].p + top, V_[d].p + bot, W32 + top, M.V[d].p + bot, st, sb, nd.kt, ld,
+                      ld, ld, ld, 0});
+        bd.push_back({U_[d].p + bot, V_[d].p + top, W32 + bot, M.V[d].p + top, sb, st, nd.kb, ld,
+                      ld, ld, ld, 0});
+      }
+    }
+    launch_balance(arena_.push(bd, stream_), (int)bd.size(), kmax, stream_);
+  }
+  if (trace_) {
+    std::fprintf(stderr, "[mixed] leaves: non-finite Linv %lld, W %lld\n",
+                 count_nonfinite(Linv_.p, leaf_elems_), count_nonfinite(M.W.p, N * ktot_));
+    for (int d = 0; d < nlev_; d++)
+      if (K_[d] > 0)
+        std::fprintf(stderr, "[mixed] level %d panels: non-finite U %lld, V %lld, V32 %lld\n", d,
+                     count_nonfinite(U_[d].p, N * K_[d]), count_nonfinite(V_[d].p, N * K_[d]),
+                     count_nonfinite(M.V[d].p, N * K_[d]));
+  }
+  const int nn = (int)nodes_.size();
+  M.c_off.assign(nn, 0);
+  M.p_off.assign(nn, 0);
+  size_t co = 0, po = 0;
+  for (int i = 0; i < nn; i++) {
+    const int K = nodes_[i].kt + nodes_[i].kb;
+    M.c_off[i] = co;
+    M.p_off[i] = po;
+    co += (size_t)K * K;
+    po += K;
+  }
+  M.C.alloc(std::max<size_t>(co, 1));
+  M.piv.alloc(std::max<size_t>(po, 1));
+  float* W = M.W.p;
+  for (int d = nlev_ - 1; d >= 0; d--) {
+    const int offd = off_[d];
+    if (offd > 0 && !level_leaves_[d].empty()) {
+      // leaf inverses applied in FP64: their condition numbers reach 1e6 on
+      // smooth kernels with small shifts, and applying an FP32-rounded inverse
+      // loses every digit of the (much smaller) result
+      size_t need = 0;
+      for (int li : level_leaves_[d]) need += (size_t)leaves_[li].s * offd;
+      grow(tmp2_, 2 * need);
+      std::vector<GemmDesc> g;
+      std::vector<ConvDesc<float, double>> up;
+      std::vector<ConvDesc<double, float>> down;
+      size_t to = 0;
+      for (int li : level_leaves_[d]) {
+        const LeafH& lf = leaves_[li];
+        double* x64 = tmp2_.p + to;
+        double* y64 = tmp2_.p + need + to;
+        up.push_back({W + lf.row0, x64, lf.s, offd, ld, lf.s});
+        g.push_back(gemm_desc(lf.s, offd, lf.s, 0, Linv_.p + lf.mat_off, lf.s, x64, lf.s, y64, lf.s,
+                              1.0, 0.0));
+        down.push_back({y64, W + lf.row0, lf.s, offd, lf.s, ld});
+        to += (size_t)lf.s * offd;
+      }
+      launch_convert(arena_.push(up, stream_), (int)up.size(), stream_);
+      run_gemm(g);
+      launch_convert(arena_.push(down, stream_), (int)down.size(), stream_);
+      if (trace_)
+        std::fprintf(stderr, "[mixed] level %d leaf apply: non-finite W %lld\n", d,
+                     count_nonfinite(M.W.p, N * ktot_));
+    }
+    if (level_nodes_[d].empty()) continue;
+    const int ncols = offd + K_[d];
+    size_t zneed = 0, yneed = 0;
+    for (int id : level_nodes_[d]) {
+      const int K = nodes_[id].kt + nodes_[id].kb;
+      zneed += (size_t)K * ncols;
+      yneed += (size_t)K * std::max(offd, 1);
+    }
+    grow_t(t32a_, zneed);
+    grow_t(t32b_, yneed);
+    grow(tmp_, yneed);
+    std::vector<GemmDescF> gz, gu;
+    std::vector<CapDescF> cd;
+    std::vector<LuDesc> lu;
+    std::vector<SolveDesc> sd;
+    std::vector<ConvDesc<double, float>> cq;
+    size_t zo = 0, yo = 0;
+    for (int id : level_nodes_[d]) {
+      const TreeNodeH& nd = nodes_[id];
+      const int kt = nd.kt, kb = nd.kb, K = kt + kb;
+      const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+      const int top = nd.row0, bot = nd.row0 + st;
+      float* Zt = t32a_.p + zo;
+      float* Zb = Zt + (size_t)kb * ncols;
+      zo += (size_t)K * ncols;
+      gz.push_back(gemm_desc<float>(kb, ncols, st, 1, M.V[d].p + top, ld, W + top, ld, Zt, kb, 1.0f,
+                                    0.0f));
+      gz.push_back(gemm_desc<float>(kt, ncols, sb, 1, M.V[d].p + bot, ld, W + bot, ld, Zb, kt, 1.0f,
+                                    0.0f));
+      double* Cn = M.C.p + M.c_off[id];
+      double* Y = tmp_.p + yo;
+      float* Q = t32b_.p + yo;
+      yo += (size_t)K * std::max(offd, 1);
+      CapDescF c{};
+      c.Zt = Zt;
+      c.Zb = Zb;
+      c.C = Cn;
+      c.Y = Y;
+      c.kt = kt;
+      c.kb = kb;
+      c.nanc = offd;
+      c.own0 = offd;
+      c.lu_order = which;
+      cd.push_back(c);
+      lu.push_back({Cn, M.piv.p + M.p_off[id], K, K});
+      if (offd > 0) {
+        SolveDesc sv{};
+        sv.LU = Cn;
+        sv.piv = M.piv.p + M.p_off[id];
+        sv.B = Y;
+        sv.n = K;
+        sv.ld_lu = K;
+        sv.ldb = K;
+        sv.ncols = offd;
+        sd.push_back(sv);
+        cq.push_back({Y, Q, K, offd, K, K});
+        const int qt = which == 0 ? 0 : kb;
+        const int qb = which == 0 ? kt : 0;
+        gu.push_back(gemm_desc<float>(st, offd, kt, 0, W + top + (size_t)offd * ld, ld, Q + qt, K,
+                                      W + top, ld, -1.0f, 1.0f));
+        gu.push_back(gemm_desc<float>(sb, offd, kb, 0, W + bot + (size_t)offd * ld, ld, Q + qb, K,
+                                      W + bot, ld, -1.0f, 1.0f));
+      }
+    }
+    run_gemm_t<float>(gz);
+    launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
+    lu_batch(lu);
+    if (offd > 0) {
+      lu_solve_batch(sd);
+      launch_convert(arena_.push(cq, stream_), (int)cq.size(), stream_);
+      run_gemm_t<float>(gu);
+    }
+    if (trace_)
+      std::fprintf(stderr, "[mixed] level %d: non-finite W %lld, Z %lld, C %lld\n", d,
+                   count_nonfinite(M.W.p, N * ktot_), count_nonfinite(t32a_.p, zneed),
+                   count_nonfinite(M.C.p, co));
+  }
+  HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
+  HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
+  float ms = 0;
+  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  stats_.factor_time = ms * 1e-3;
+  arena_.reset();
+  M.done = true;
+}
+
+/* One application of the mixed factorization: X <- M^-1 X (FP32, in place). */
+void Engine::solve_mixed_once(int which, float* X, int nrhs) {
+  MixedFactorization& M = mfac_[which];
+  const int ld = ntot_;
+  const size_t N = (size_t)ntot_;
+  {
+    // leaf inverses in FP64 (see factorize_mixed)
+    const size_t need = N * nrhs;
+    grow(tmp2_, 2 * need);
+    std::vector<GemmDesc> g;
+    std::vector<ConvDesc<float, double>> up;
+    std::vector<ConvDesc<double, float>> down;
+    size_t to = 0;
+    for (auto& lf : leaves_) {
+      double* x64 = tmp2_.p + to;
+      double* y64 = tmp2_.p + need + to;
+      up.push_back({X + lf.row0, x64, lf.s, nrhs, ld, lf.s});
+      g.push_back(gemm_desc(lf.s, nrhs, lf.s, 0, Linv_.p + lf.mat_off, lf.s, x64, lf.s, y64, lf.s,
+                            1.0, 0.0));
+      down.push_back({y64, X + lf.row0, lf.s, nrhs, lf.s, ld});
+      to += (size_t)lf.s * nrhs;
+    }
+    launch_convert(arena_.push(up, stream_), (int)up.size(), stream_);
+    run_gemm(g);
+    launch_convert(arena_.push(down, stream_), (int)down.size(), stream_);
+  }
+  for (int d = nlev_ - 1; d >= 0; d--) {
+    if (level_nodes_[d].empty()) continue;
+    const int offd = off_[d];
+    size_t zneed = 0;
+    for (int id : level_nodes_[d]) zneed += (size_t)(nodes_[id].kt + nodes_[id].kb) * nrhs;
+    grow_t(t32a_, zneed);
+    grow_t(t32b_, zneed);
+    grow(tmp_, zneed);
+    std::vector<GemmDescF> gz, gu;
+    std::vector<CapDescF> cd;
+    std::vector<SolveDesc> sd;
+    std::vector<ConvDesc<double, float>> cq;
+    size_t zo = 0;
+    for (int id : level_nodes_[d]) {
+      const TreeNodeH& nd = nodes_[id];
+      const int kt = nd.kt, kb = nd.kb, K = kt + kb;
+      const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+      const int top = nd.row0, bot = nd.row0 + st;
+      float* Zt = t32a_.p + zo;
+      float* Zb = Zt + (size_t)kb * nrhs;
+      double* Y = tmp_.p + zo;
+      float* Q = t32b_.p + zo;
+      zo += (size_t)K * nrhs;
+      gz.push_back(gemm_desc<float>(kb, nrhs, st, 1, M.V[d].p + top, ld, X + top, ld, Zt, kb, 1.0f, 0.0f));
+      gz.push_back(gemm_desc<float>(kt, nrhs, sb, 1, M.V[d].p + bot, ld, X + bot, ld, Zb, kt, 1.0f, 0.0f));
+      CapDescF c{};
+      c.Zt = Zt;
+      c.Zb = Zb;
+      c.C = nullptr;
+      c.Y = Y;
+      c.kt = kt;
+      c.kb = kb;
+      c.nanc = nrhs;
+      c.own0 = 0;
+      c.lu_order = which;
+      cd.push_back(c);
+      SolveDesc sv{};
+      sv.LU = M.C.p + M.c_off[id];
+      sv.piv = M.piv.p + M.p_off[id];
+      sv.B = Y;
+      sv.n = K;
+      sv.ld_lu = K;
+      sv.ldb = K;
+      sv.ncols = nrhs;
+      sd.push_back(sv);
+      cq.push_back({Y, Q, K, nrhs, K, K});
+      const int qt = which == 0 ? 0 : kb;
+      const int qb = which == 0 ? kt : 0;
+      gu.push_back(gemm_desc<float>(st, nrhs, kt, 0, M.W.p + top + (size_t)offd * ld, ld, Q + qt, K,
+                                    X + top, ld, -1.0f, 1.0f));
+      gu.push_back(gemm_desc<float>(sb, nrhs, kb, 0, M.W.p + bot + (size_t)offd * ld, ld, Q + qb, K,
+                                    X + bot, ld, -1.0f, 1.0f));
+    }
+    run_gemm_t<float>(gz);
+    launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
+    lu_solve_batch(sd);
+    launch_convert(arena_.push(cq, stream_), (int)cq.size(), stream_);
+    run_gemm_t<float>(gu);
+  }
+}
+
+/* Refinement by restarted flexible GMRES (FGMRES-IR): solves A_H x = b with
+ * the FP64 compressed operator A_H (leaves plus U V^T) and the preconditioner
+ * M^-1 = the mixed-precision factorization (mixed) or the FP64 one, until
+ * every column satisfies ||b - A_H x|| <= target ||b|| (true residual in
+ * double-double arithmetic, checked at each restart) or the step budget is
+ * spent.  The preconditioned vectors
+ * z_j = M^-1 v_j are kept and the update is x += Z y: an FP32 preconditioner
+ * is not exactly linear, and applying it again to V y would break the
+ * agreement between the Arnoldi residual estimate and the true residual.
+ * Columns run in lockstep: one preconditioner application and one operator
+ * product per step serve all right-hand sides; orthogonalization is classical
+ * Gram-Schmidt applied twice, per column. */
+void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int nrhs, double* dX,
+                         int ldx) {
+  if (mixed)
+    factorize_mixed(which);
+  else
+    factorize_fp64(which);
+  HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
+  const int ld = ntot_;
+  const size_t N = (size_t)ntot_;
+  const int m = std::max(1, std::min(gmres_restart_, std::max(1, refine_max_)));
+  grow(xw_, N * nrhs);
+  grow(rbuf_, N * nrhs);
+  grow(basis_, (size_t)(m + 1) * N * nrhs);
+  grow(zbasis_, (size_t)m * N * nrhs);
+  grow(nbuf_, (size_t)(m + 2) * nrhs + nrhs);
+  if (mixed) grow_t(x32_, N * nrhs);
+  const size_t vstride = N * nrhs;
+  double* V = basis_.p;
+  double* dcoef = nbuf_.p;            // (m + 1) x nrhs coefficients
+  double* dnorm = nbuf_.p + (size_t)(m + 1) * nrhs;  // nrhs values
+  auto colnorms = [&](const double* X, std::vector<double>& out) {
+    launch_colnorm2(X, ntot_, ld, nrhs, dnorm, stream_);
+    out.resize(nrhs);
+    HODLR_HIP_CHECK(hipMemcpyAsync(out.data(), dnorm, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (auto& v : out) v = std::sqrt(v);
+  };
+  auto upload = [&](const std::vector<double>& h, double* d) {
+    HODLR_HIP_CHECK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));  // h is reused right away
+  };
+  auto precond = [&](const double* in, double* out) {  // out = M^-1 in
+    if (mixed) {
+      std::vector<ConvDesc<double, float>> cv{{in, x32_.p, ntot_, nrhs, ld, ld}};
+      launch_convert(arena_.push(cv, stream_), 1, stream_);
+      solve_mixed_once(which, x32_.p, nrhs);
+      std::vector<ConvDesc<float, double>> cb{{x32_.p, out, ntot_, nrhs, ld, ld}};
+      launch_convert(arena_.push(cb, stream_), 1, stream_);
+    } else {
+      if (in != out)
+        HODLR_HIP_CHECK(hipMemcpyAsync(out, in, N * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+      solve_fp64_inplace(which, out, nrhs);
+    }
+  };
+  std::vector<double> bn, rn, hn, coef((size_t)(m + 1) * nrhs);
+  HODLR_HIP_CHECK(hipMemsetAsync(xw_.p, 0, N * nrhs * sizeof(double), stream_));
+  HODLR_HIP_CHECK(hipMemcpy2DAsync(rbuf_.p, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
+                                   N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
+  colnorms(rbuf_.p, bn);
+  for (auto& v : bn)
+    if (!(v > 0)) v = 1.0;
+  // Hessenberg, Givens rotations and right-hand side per column (host)
+  std::vector<double> H((size_t)(m + 1) * m * nrhs), cs((size_t)m * nrhs), sn((size_t)m * nrhs),
+      g((size_t)(m + 1) * nrhs);
+  auto Hc = [&](int c) { return H.data() + (size_t)c * (m + 1) * m; };
+  int total = 0;
+  double res = 0.0;
+  for (;;) {
+    colnorms(rbuf_.p, rn);
+    res = 0.0;
+    for (int c = 0; c < nrhs; c++) {
+      const double rc = rn[c] / bn[c];
+      if (!(rc <= res)) res = rc;
+    }
+    if (trace_) std::fprintf(stderr, "[gmres] steps %d true residual %.3e\n", total, res);
+    if (!std::isfinite(res)) throw std::runtime_error("refinement produced a non-finite residual");
+    if (res <= refine_target_ || total >= refine_max_) break;
+    // V_0 = r / ||r||; converged columns get a zero basis and stay put
+    std::vector<double> inv(nrhs);
+    std::vector<char> active(nrhs);
+    for (int c = 0; c < nrhs; c++) {
+      active[c] = rn[c] / bn[c] > refine_target_ && rn[c] > 0;
+      inv[c] = active[c] ? 1.0 / rn[c] : 0.0;
+      g[(size_t)c * (m + 1)] = active[c] ? rn[c] : 0.0;
+      for (int i = 1; i <= m; i++) g[(size_t)c * (m + 1) + i] = 0.0;
+    }
+    HODLR_HIP_CHECK(hipMemcpyAsync(V, rbuf_.p, N * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    upload(inv, dnorm);
+    launch_col_scale(V, ld, ntot_, nrhs, dnorm, stream_);
+    int jend = 0;
+    for (int j = 0; j < m; j++) {
+      double* Vj = V + (size_t)j * vstride;
+      double* Vn = V + (size_t)(j + 1) * vstride;
+      double* Zj = zbasis_.p + (size_t)j * vstride;
+      precond(Vj, Zj);
+      matvec_device(Zj, ld, nrhs, Vn, ld);
+      // two passes of classical Gram-Schmidt against V_0..V_j
+      std::vector<double> h((size_t)(j + 1) * nrhs, 0.0);
+      for (int pass = 0; pass < 2; pass++) {
+        launch_multi_dot(V, vstride, j + 1, Vn, ntot_, ld, nrhs, dcoef, stream_);
+        HODLR_HIP_CHECK(hipMemcpyAsync(coef.data(), dcoef, (size_t)(j + 1) * nrhs * sizeof(double),
+                                       hipMemcpyDeviceToHost, stream_));
+        HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+        launch_multi_axpy(V, vstride, j + 1, dcoef, Vn, ntot_, ld, nrhs, -1.0, 0, stream_);
+        for (size_t e = 0; e < h.size(); e++) h[e] += coef[e];
+      }
+      colnorms(Vn, hn);
+      for (int c = 0; c < nrhs; c++) {
+        inv[c] = (active[c] && hn[c] > 0) ? 1.0 / hn[c] : 0.0;
+        double* Hj = Hc(c) + (size_t)j * (m + 1);  // column j of H, m+1 rows
+        for (int i = 0; i <= j; i++) Hj[i] = active[c] ? h[i + (size_t)c * (j + 1)] : 0.0;
+        Hj[j + 1] = active[c] ? hn[c] : 0.0;
+        if (!active[c]) {
+          cs[(size_t)c * m + j] = 1.0;
+          sn[(size_t)c * m + j] = 0.0;
+          Hj[j] = 1.0;  // keeps the triangular solve regular; g stays 0
+          continue;
+        }
+        for (int i = 0; i < j; i++) {  // previous rotations
+          const double a = Hj[i], b = Hj[i + 1];
+          const double cc = cs[(size_t)c * m + i], ss = sn[(size_t)c * m + i];
+          Hj[i] = cc * a + ss * b;
+          Hj[i + 1] = -ss * a + cc * b;
+        }
+        const double a = Hj[j], b = Hj[j + 1];
+        const double r = std::hypot(a, b);
+        const double cc = r > 0 ? a / r : 1.0, ss = r > 0 ? b / r : 0.0;
+        cs[(size_t)c * m + j] = cc;
+        sn[(size_t)c * m + j] = ss;
+        Hj[j] = r;
+        Hj[j + 1] = 0.0;
+        double* gc = g.data() + (size_t)c * (m + 1);
+        gc[j + 1] = -ss * gc[j];
+        gc[j] = cc * gc[j];
+        if (!(hn[c] > 0)) active[c] = 0;  // happy breakdown: exact in this space
+      }
+      upload(inv, dnorm);
+      launch_col_scale(Vn, ld, ntot_, nrhs, dnorm, stream_);
+      total++;
+      jend = j + 1;
+      double est = 0.0;
+      for (int c = 0; c < nrhs; c++) est = std::max(est, std::fabs(g[(size_t)c * (m + 1) + j + 1]) / bn[c]);
+      if (trace_) std::fprintf(stderr, "[gmres] step %d estimate %.3e\n", total, est);
+      if (est <= 0.5 * refine_target_ || total >= refine_max_) break;
+    }
+    // y = H^-1 g per column, x += Z y
+    std::fill(coef.begin(), coef.end(), 0.0);
+    for (int c = 0; c < nrhs; c++) {
+      const double* Hm = Hc(c);
+      const double* gc = g.data() + (size_t)c * (m + 1);
+      double* y = coef.data() + (size_t)c * jend;
+      for (int i = jend - 1; i >= 0; i--) {
+        double v = gc[i];
+        for (int k = i + 1; k < jend; k++) v -= Hm[(size_t)k * (m + 1) + i] * y[k];
+        const double d = Hm[(size_t)i * (m + 1) + i];
+        y[i] = d != 0.0 ? v / d : 0.0;
+      }
+    }
+    std::vector<double> yv(coef.begin(), coef.begin() + (size_t)jend * nrhs);
+    upload(yv, dcoef);
+    launch_multi_axpy(zbasis_.p, vstride, jend, dcoef, xw_.p, ntot_, ld, nrhs, 1.0, 0, stream_);
+    residual_dd(dB, ldb, xw_.p, ld, nrhs, rbuf_.p, ld);
+  }
+  stats_.refine_iterations = total;
+  stats_.last_residual = res;
+  HODLR_HIP_CHECK(hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), xw_.p, N * sizeof(double),
+                                   N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
+  HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
+  HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
+  float ms = 0;
+  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
+  stats_.solve_time = ms * 1e-3;
+  stats_.refine_time = ms * 1e-3;
+  arena_.reset();
 }
 
 /* calcDeterminant (HODLR_Matrix.cpp:1671-1697): sum of log|diag LU| over the
@@ -988,6 +1518,122 @@ void Engine::matvec_device(const double* dX, int ldx, int nrhs, double* dY, int 
   arena_.reset();
 }
 
+/* R = B - A_H X in double-double arithmetic, rounded once (ddres.h): the
+ * leaves' products first, then per level the projections T = V^T X of every
+ * node half (one launch for all levels, chunk partials reduced in order) and
+ * the updates Y += U T. */
+void Engine::residual_dd(const double* dB, int ldb, const double* dX, int ldx, int nrhs, double* dR,
+                         int ldr) {
+  if (nrhs <= 0) return;
+  compress();
+  const size_t N = (size_t)ntot_;
+  const int ld = ntot_;
+  grow(ddy_, 2 * N * nrhs);
+  double* yh = ddy_.p;
+  double* yl = ddy_.p + N * nrhs;
+  // projections of all levels
+  std::vector<DdProjDesc> pd;
+  std::vector<std::pair<int, int>> pd_node;  // (node, half) of each projection
+  long long poff = 0, toff = 0;
+  int pwg = 0, rwg = 0;
+  for (int id = 0; id < (int)nodes_.size(); id++) {
+    const TreeNodeH& nd = nodes_[id];
+    const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+    const int top = nd.row0, bot = nd.row0 + st;
+    for (int h = 0; h < 2; h++) {
+      const int k = h == 0 ? nd.kt : nd.kb;
+      if (k == 0) continue;
+      DdProjDesc p{};
+      // top block U_t V_t^T: V_t on the bottom rows; bottom block the reverse
+      p.V = V_[nd.level].p + (h == 0 ? bot : top);
+      p.X = dX + (h == 0 ? bot : top);
+      p.ldv = ld;
+      p.ldx = ldx;
+      p.rows = h == 0 ? sb : st;
+      p.k = k;
+      p.nrhs = nrhs;
+      p.nchunks = (p.rows + DD_CHUNK - 1) / DD_CHUNK;
+      p.wg0 = pwg;
+      p.rwg0 = rwg;
+      p.part_off = poff;
+      p.t_off = toff;
+      pwg += dd_proj_wgs(p);
+      rwg += dd_reduce_wgs(p);
+      poff += (long long)p.nchunks * k * nrhs;
+      toff += (long long)k * nrhs;
+      pd.push_back(p);
+      pd_node.push_back({id, h});
+    }
+  }
+  grow(ddpart_, 2 * std::max<long long>(poff, 1));
+  grow(ddt_, 2 * std::max<long long>(toff, 1));
+  double* ph = ddpart_.p;
+  double* pl = ddpart_.p + std::max<long long>(poff, 1);
+  double* th = ddt_.p;
+  double* tl = ddt_.p + std::max<long long>(toff, 1);
+  if (!pd.empty()) {
+    const DdProjDesc* dpd = arena_.push(pd, stream_);
+    launch_dd_proj(dpd, (int)pd.size(), pwg, ph, pl, stream_);
+    launch_dd_reduce(dpd, (int)pd.size(), rwg, ph, pl, th, tl, stream_);
+  }
+  // leaves: Y = L X
+  {
+    std::vector<DdApplyDesc> ad;
+    int wg = 0;
+    for (auto& lf : leaves_) {
+      DdApplyDesc a{};
+      a.A = Lorig_.p + lf.mat_off;
+      a.lda = lf.s;
+      a.t_hi = dX + lf.row0;
+      a.t_lo = nullptr;
+      a.ldt = ldx;
+      a.y_hi = yh + lf.row0;
+      a.y_lo = yl + lf.row0;
+      a.ldy = ld;
+      a.rows = lf.s;
+      a.k = lf.s;
+      a.nrhs = nrhs;
+      a.accumulate = 0;
+      a.wg0 = wg;
+      wg += dd_apply_wgs(a);
+      ad.push_back(a);
+    }
+    launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, stream_);
+  }
+  // low-rank updates, level by level (levels overlap in rows)
+  for (int d = 0; d < nlev_; d++) {
+    std::vector<DdApplyDesc> ad;
+    int wg = 0;
+    for (size_t q = 0; q < pd.size(); q++) {
+      const TreeNodeH& nd = nodes_[pd_node[q].first];
+      if (nd.level != d) continue;
+      const int h = pd_node[q].second;
+      const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+      const int row = h == 0 ? nd.row0 : nd.row0 + st;
+      DdApplyDesc a{};
+      a.A = U_[d].p + row;
+      a.lda = ld;
+      a.t_hi = th + pd[q].t_off;
+      a.t_lo = tl + pd[q].t_off;
+      a.ldt = pd[q].k;
+      a.y_hi = yh + row;
+      a.y_lo = yl + row;
+      a.ldy = ld;
+      a.rows = h == 0 ? st : sb;
+      a.k = pd[q].k;
+      a.nrhs = nrhs;
+      a.accumulate = 1;
+      a.wg0 = wg;
+      wg += dd_apply_wgs(a);
+      ad.push_back(a);
+    }
+    if (!ad.empty()) launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, stream_);
+  }
+  launch_dd_residual(dB, ldb, yh, yl, ld, dR, ldr, ntot_, nrhs, stream_);
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  arena_.reset();
+}
+
 void Engine::get_factor(int idx, int which, double* host_out) {
   compress();
   if (idx < 0 || idx >= (int)nodes_.size() || which < 0 || which > 3)
@@ -1009,6 +1655,6 @@ void Engine::get_factor(int idx, int which, double* host_out) {
                               hipMemcpyDeviceToHost));
 }
 
-template double* DescArena::push<double>(const std::vector<double>&, hipStream_t);
+
 
 }  // namespace hodlr

@@ -27,6 +27,7 @@
 
 #include "aca.h"
 #include "common.h"
+#include "ddres.h"
 #include "dense.h"
 #include "gemm.h"
 
@@ -117,6 +118,18 @@ struct LeafH {
   size_t piv_off;
 };
 
+/* Mixed-precision factorization: FP32 U^SM and V panels for the bulk GEMMs,
+ * FP64 node matrices (LU) and leaf LU. */
+struct MixedFactorization {
+  bool done = false;
+  DBuf<float> W;
+  std::vector<DBuf<float>> V;
+  DBuf<double> C;
+  DBuf<int> piv;
+  std::vector<size_t> c_off;
+  std::vector<size_t> p_off;
+};
+
 struct Factorization {
   bool done = false;
   DBuf<double> W;
@@ -146,6 +159,8 @@ class Engine {
   void solve_host(int which, const double* B, int ldb, int nrhs, double* X, int ldx);
   double logabsdet(double* det_prod);
   void matvec_device(const double* dX, int ldx, int nrhs, double* dY, int ldy);
+  void residual_dd(const double* dB, int ldb, const double* dX, int ldx, int nrhs, double* dR,
+                   int ldr);
 
   // inspection
   int rows() const { return ntot_; }
@@ -158,7 +173,11 @@ class Engine {
     double compress_time = 0, factor_time = 0, solve_time = 0, refine_time = 0;
     int refine_iterations = 0;
     double last_residual = 0;
+    double aca_row_time = 0, aca_col_time = 0;  // with profiling on
+    int aca_launches = 0;
+    double aca_bytes = 0;
   };
+  void set_profiling(bool on) { profiling_ = on; }
   const Stats& stats() const { return stats_; }
   long long device_bytes() const;
   hipStream_t stream() const { return stream_; }
@@ -171,9 +190,20 @@ class Engine {
   void assemble_leaves();
   void factor_leaves();
   void run_gemm(std::vector<GemmDesc>& descs);
+  template <class T>
+  void run_gemm_t(std::vector<GemmDescT<T>>& descs);
   void ensure_ws(size_t doubles);
   void grow(DBuf<double>& b, size_t doubles);
   void lu_batch(std::vector<LuDesc>& descs);
+  void factorize_fp64(int which);
+  void factorize_mixed(int which);
+  void solve_fp64_inplace(int which, double* X, int nrhs);
+  void solve_mixed_once(int which, float* X, int nrhs);
+  void solve_gmres(int which, bool mixed, const double* dB, int ldb, int nrhs, double* dX, int ldx);
+  template <class T>
+  void grow_t(DBuf<T>& b, size_t n);
+  template <class T>
+  long long count_nonfinite(const T* x, size_t n);
   void lu_solve_batch(std::vector<SolveDesc>& descs);
 
   std::vector<InstanceSpec> specs_;
@@ -191,6 +221,9 @@ class Engine {
   double refine_target_ = 1e-10;
   int refine_max_ = 30;
   int init_cap_ = 0;
+  bool profiling_ = false;
+  bool trace_ = false;
+  std::vector<hipEvent_t> prof_ev_;
 
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
@@ -209,8 +242,13 @@ class Engine {
   DBuf<int> Lpiv_;
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;
   Factorization fac_[2];
+  MixedFactorization mfac_[2];
+  DBuf<float> t32a_, t32b_, x32_;
+  DBuf<double> rbuf_, ybuf_, zbuf_, nbuf_, basis_, zbasis_;
+  int gmres_restart_ = 20;
   DBuf<double> ws_, tmp_, tmp2_;
   DBuf<double> xw_;
+  DBuf<double> ddy_, ddpart_, ddt_;  // double-double residual: Y, partials, projections
   Stats stats_;
 };
 

@@ -11,14 +11,17 @@
 namespace hodlr {
 
 /* Tile height (128 or 64) that suits a group of products. */
-int gemm_tile_m(const std::vector<GemmDesc>& descs);
+template <class T>
+int gemm_tile_m(const std::vector<GemmDescT<T>>& descs);
 
 /* Assigns tile offsets and K splits for tiles bm x 64; returns the workspace
- * size in doubles. */
-size_t gemm_plan(std::vector<GemmDesc>& descs, std::vector<int>& split_ids, int& ntiles,
+ * size in elements. */
+template <class T>
+size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, int& ntiles,
                  int target_tiles, int bm);
 
-void launch_gemm(const GemmDesc* d_descs, int count, int ntiles, double* d_ws,
+template <class T>
+void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s);
 
 }  // namespace hodlr

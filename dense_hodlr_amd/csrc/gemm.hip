@@ -26,8 +26,8 @@ namespace {
 
 constexpr int BK = 16;
 
-template <int BM, int BN>
-__device__ __forceinline__ int find_problem(const GemmDesc* __restrict__ descs, int count) {
+template <class D>
+__device__ __forceinline__ int find_problem(const D* __restrict__ descs, int count) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -39,17 +39,17 @@ __device__ __forceinline__ int find_problem(const GemmDesc* __restrict__ descs, 
   return lo;
 }
 
-template <int BM, int BN>
+template <class T, int BM, int BN>
 __global__ void __launch_bounds__((BM / 8) * (BN / 4))
-    gemm_kernel(const GemmDesc* __restrict__ descs, int count, double* __restrict__ ws) {
+    gemm_kernel(const GemmDescT<T>* __restrict__ descs, int count, T* __restrict__ ws) {
   constexpr int TM = 8, TN = 4;
   constexpr int NT = (BM / TM) * (BN / TN);
   constexpr int RX = BM / TM;  // threads along M
   constexpr int A_PER = BM * BK / NT;
   constexpr int B_PER = BN * BK / NT;
-  __shared__ double As[2][BK][BM + 1];
-  __shared__ double Bs[2][BK][BN + 1];
-  const GemmDesc g = descs[find_problem<BM, BN>(descs, count)];
+  __shared__ T As[2][BK][BM + 1];
+  __shared__ T Bs[2][BK][BN + 1];
+  const GemmDescT<T> g = descs[find_problem(descs, count)];
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   int tile = blockIdx.x - g.tile0;
   const int split = tile / (tm * tn);
@@ -59,13 +59,13 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   const int kbeg = split * kchunk, kend = min(g.K, kbeg + kchunk);
   const int t = threadIdx.x, tx = t % RX, ty = t / RX;
 
-  double acc[TM][TN];
+  T acc[TM][TN];
 #pragma unroll
   for (int a = 0; a < TM; a++)
 #pragma unroll
-    for (int b = 0; b < TN; b++) acc[a][b] = 0.0;
+    for (int b = 0; b < TN; b++) acc[a][b] = T(0);
 
-  double ra[A_PER], rb[B_PER];
+  T ra[A_PER], rb[B_PER];
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < A_PER; r++) {
@@ -81,14 +81,14 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
       const int gm = m0 + mm, gk = k0 + kk;
       ra[r] = (gm < g.M && gk < kend)
                   ? (g.trans_a ? g.A[(size_t)gk + (size_t)gm * g.lda] : g.A[(size_t)gm + (size_t)gk * g.lda])
-                  : 0.0;
+                  : T(0);
     }
 #pragma unroll
     for (int r = 0; r < B_PER; r++) {
       const int e = t + r * NT;
       const int kk = e % BK, nn = e / BK;
       const int gn = n0 + nn, gk = k0 + kk;
-      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : 0.0;
+      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : T(0);
     }
   };
   auto store = [&](int buf) {
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
     if (more) load(k0 + BK);
 #pragma unroll 2
     for (int kk = 0; kk < BK; kk++) {
-      double av[TM], bv[TN];
+      T av[TM], bv[TN];
 #pragma unroll
       for (int a = 0; a < TM; a++) av[a] = As[buf][kk][tx + RX * a];
 #pragma unroll
@@ -144,12 +144,12 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
       for (int b = 0; b < TN; b++) {
         const int gm = m0 + tx + RX * a, gn = n0 + ty + (BN / TN) * b;
         if (gm < g.M && gn < g.N) {
-          double* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
-          *c = (g.beta == 0.0) ? g.alpha * acc[a][b] : g.alpha * acc[a][b] + g.beta * *c;
+          T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+          *c = (g.beta == T(0)) ? g.alpha * acc[a][b] : g.alpha * acc[a][b] + g.beta * *c;
         }
       }
   } else {
-    double* w = ws + g.ws_off + (size_t)split * g.M * g.N;
+    T* w = ws + g.ws_off + (size_t)split * g.M * g.N;
 #pragma unroll
     for (int a = 0; a < TM; a++)
 #pragma unroll
@@ -160,23 +160,25 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(256)
-    splitk_reduce_kernel(const GemmDesc* __restrict__ descs, const int* __restrict__ ids,
-                         const double* __restrict__ ws) {
-  const GemmDesc g = descs[ids[blockIdx.y]];
+    splitk_reduce_kernel(const GemmDescT<T>* __restrict__ descs, const int* __restrict__ ids,
+                         const T* __restrict__ ws) {
+  const GemmDescT<T> g = descs[ids[blockIdx.y]];
   const size_t mn = (size_t)g.M * g.N;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < mn; e += (size_t)gridDim.x * 256) {
-    double s = 0.0;
+    T s = T(0);
     for (int sp = 0; sp < g.ksplit; sp++) s += ws[g.ws_off + (size_t)sp * mn + e];
     const int gm = (int)(e % g.M), gn = (int)(e / g.M);
-    double* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
-    *c = (g.beta == 0.0) ? g.alpha * s : g.alpha * s + g.beta * *c;
+    T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+    *c = (g.beta == T(0)) ? g.alpha * s : g.alpha * s + g.beta * *c;
   }
 }
 
 }  // namespace
 
-int gemm_tile_m(const std::vector<GemmDesc>& descs) {
+template <class T>
+int gemm_tile_m(const std::vector<GemmDescT<T>>& descs) {
   // the wide tile when most of the work has more than 64 rows
   double big = 0, small = 0;
   for (auto& g : descs) {
@@ -189,7 +191,8 @@ int gemm_tile_m(const std::vector<GemmDesc>& descs) {
   return big >= small ? 128 : 64;
 }
 
-size_t gemm_plan(std::vector<GemmDesc>& descs, std::vector<int>& split_ids, int& ntiles,
+template <class T>
+size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, int& ntiles,
                  int target_tiles, int bm) {
   const int BM = bm, BN = 64;
   size_t tiles_out = 0;
@@ -201,7 +204,7 @@ size_t gemm_plan(std::vector<GemmDesc>& descs, std::vector<int>& split_ids, int&
   int tile = 0;
   split_ids.clear();
   for (size_t p = 0; p < descs.size(); p++) {
-    GemmDesc& g = descs[p];
+    GemmDescT<T>& g = descs[p];
     const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
     int ks = 1;
     if (g.K >= 4 * 256 && tiles_out < (size_t)target_tiles) {
@@ -224,19 +227,29 @@ size_t gemm_plan(std::vector<GemmDesc>& descs, std::vector<int>& split_ids, int&
   return ws;
 }
 
-void launch_gemm(const GemmDesc* d_descs, int count, int ntiles, double* d_ws,
+template <class T>
+void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s) {
   if (count == 0 || ntiles == 0) return;
   if (bm == 128)
-    gemm_kernel<128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+    gemm_kernel<T, 128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
   else
-    gemm_kernel<64, 64><<<ntiles, 128, 0, s>>>(d_descs, count, d_ws);
+    gemm_kernel<T, 64, 64><<<ntiles, 128, 0, s>>>(d_descs, count, d_ws);
   HODLR_HIP_CHECK(hipGetLastError());
   if (nsplit > 0) {
     dim3 g(64, nsplit);
-    splitk_reduce_kernel<<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
+    splitk_reduce_kernel<T><<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
     HODLR_HIP_CHECK(hipGetLastError());
   }
 }
+
+template int gemm_tile_m<double>(const std::vector<GemmDesc>&);
+template int gemm_tile_m<float>(const std::vector<GemmDescF>&);
+template size_t gemm_plan<double>(std::vector<GemmDesc>&, std::vector<int>&, int&, int, int);
+template size_t gemm_plan<float>(std::vector<GemmDescF>&, std::vector<int>&, int&, int, int);
+template void launch_gemm<double>(const GemmDesc*, int, int, double*, const int*, int, int,
+                                  hipStream_t);
+template void launch_gemm<float>(const GemmDescF*, int, int, float*, const int*, int, int,
+                                 hipStream_t);
 
 }  // namespace hodlr

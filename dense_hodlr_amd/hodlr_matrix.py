@@ -100,6 +100,9 @@ class HODLR_Matrix:
     def set_precision(self, precision: str):
         check(lib().hodlr_set_precision(self._h, _lib.PRECISIONS[precision]), self._h)
 
+    def set_profiling(self, on: bool = True):
+        check(lib().hodlr_set_profiling(self._h, int(bool(on))), self._h)
+
     def set_refinement(self, target: float = 1e-10, max_iters: int = 30):
         check(lib().hodlr_set_refinement(self._h, float(target), int(max_iters)), self._h)
 
@@ -142,6 +145,17 @@ class HODLR_Matrix:
         check(lib().hodlr_matvec(self._h, _dptr(X), self._n, X.shape[1], _dptr(Y), self._n), self._h)
         return Y
 
+    def residual(self, B, X):
+        """B - A_H X in double-double arithmetic, rounded once."""
+        B = _as_f64_2d(B, self._n)
+        X = _as_f64_2d(X, self._n)
+        if B.shape[1] != X.shape[1]:
+            raise ValueError("B and X have different numbers of columns")
+        R = np.zeros_like(B, order="F")
+        check(lib().hodlr_residual(self._h, _dptr(B), self._n, B.shape[1], _dptr(X), self._n,
+                                   _dptr(R), self._n), self._h)
+        return R
+
     # ---- accessors -------------------------------------------------------
     def rows(self) -> int:
         return self._n
@@ -158,7 +172,9 @@ class HODLR_Matrix:
                     factor_time=s.factor_time, solve_time=s.solve_time,
                     refine_time=s.refine_time, refine_iterations=s.refine_iterations,
                     last_residual=s.last_residual, max_rank=list(s.max_rank[:L]),
-                    mean_rank=list(s.mean_rank[:L]), device_bytes=s.device_bytes)
+                    mean_rank=list(s.mean_rank[:L]), device_bytes=s.device_bytes,
+                    aca_row_time=s.aca_row_time, aca_col_time=s.aca_col_time,
+                    aca_launches=s.aca_launches, aca_bytes=s.aca_bytes)
 
     def get_LR_ComputationTime(self) -> float:
         return self.stats()["compress_time"]

@@ -54,8 +54,14 @@ enum { HODLR_DIAG_ADD_SHIFT = 0, HODLR_DIAG_REPLACE_VALUE = 1 };
  * src/HODLR_Matrix.cpp:332, :356). */
 enum { HODLR_LR_ACA = 0, HODLR_LR_SVD = 1 };
 
-/* Arithmetic of the factorization. */
-enum { HODLR_PREC_FP64 = 0, HODLR_PREC_MIXED = 1 };
+/* Arithmetic of the factorization and solve.
+ *  FP64          the reference's recSM / recLU in FP64 (no refinement);
+ *  MIXED         FP32 bulk (U^SM, V panels, their GEMMs), FP64 leaf LU,
+ *                node matrices and residuals, GMRES-based iterative
+ *                refinement against the FP64 HODLR operator;
+ *  FP64_REFINED  the FP64 factorization used as the preconditioner of the
+ *                same refinement. */
+enum { HODLR_PREC_FP64 = 0, HODLR_PREC_MIXED = 1, HODLR_PREC_FP64_REFINED = 2 };
 
 /* Which factorization (recSM: Woodbury perturbI nodes; recLU: block LU). */
 enum { HODLR_FACTOR_SM = 0, HODLR_FACTOR_LU = 1 };
@@ -90,6 +96,13 @@ typedef struct hodlr_stats {
   int max_rank[64];       /* per level */
   double mean_rank[64];   /* per level */
   long long device_bytes; /* device memory held by the handle */
+  /* with hodlr_set_profiling(h, 1): summed HIP-event durations of the ACA
+   * row-phase and column-phase launches, their count (iterations), and the
+   * algorithmic bytes of those residual passes, 8 (m+n) k(k+1)/2 per block */
+  double aca_row_time;
+  double aca_col_time;
+  int aca_launches;
+  double aca_bytes;
 } hodlr_stats;
 
 /* ---- construction ------------------------------------------------------ */
@@ -123,13 +136,17 @@ int hodlr_set_tolerance(hodlr_handle h, double tol);
  * parameter is an int in the reference (include/lowRank.hpp:25): the value is
  * truncated toward zero. */
 int hodlr_set_min_pivot(hodlr_handle h, double min_pivot);
-/* Factorization arithmetic: FP64 (reference) or mixed (FP32 bulk, FP64 leaves,
- * capacitance systems and residuals, iterative refinement).  No reference
- * counterpart. */
+/* Factorization arithmetic (HODLR_PREC_*).  No reference counterpart. */
 int hodlr_set_precision(hodlr_handle h, int precision);
-/* Mixed precision: relative residual the refinement must reach (default 1e-10)
- * and the maximum number of refinement steps (default 30). */
+/* Refinement (MIXED, FP64_REFINED): relative residual ||b - A_H x|| / ||b||
+ * every column must reach (default 1e-10; the residual is evaluated in
+ * double-double arithmetic, as hodlr_residual does) and the maximum number of
+ * flexible GMRES steps (default 30; restart length min(20, max_iters)). */
 int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
+
+/* Record HIP events around the ACA residual launches (for roofline figures);
+ * off by default.  No reference counterpart. */
+int hodlr_set_profiling(hodlr_handle h, int on);
 
 /* ---- compute ----------------------------------------------------------- */
 
@@ -153,6 +170,12 @@ int hodlr_determinant(hodlr_handle h, double* out);
 /* Y = A_H X with the compressed FP64 operator (leaves plus U V^T, the
  * operator that fill_Block reconstructs, HODLR_Matrix.cpp:1404-1480). */
 int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, int ldy);
+
+/* R = B - A_H X evaluated in double-double arithmetic (error-free products
+ * and sums) and rounded once: the residual that refinement targets are
+ * checked with.  No reference counterpart. */
+int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const double* X, int ldx,
+                   double* R, int ldr);
 
 /* ---- inspection -------------------------------------------------------- */
 

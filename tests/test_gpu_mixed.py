@@ -1,0 +1,83 @@
+"""GPU tests of the refined paths: the mixed-precision factorization (FP32
+node products, FP64 leaf inverses, node matrices and residuals) and the FP64
+one, each used as the preconditioner of flexible GMRES against the FP64 HODLR
+operator A_H until the requested relative residual is reached."""
+import sys
+
+import numpy as np
+import pytest
+
+from dense_hodlr_amd import HODLR_Matrix
+from dense_hodlr_amd import inputs as I
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def colres(g, x, b):
+    """max over columns of ||b - A_H x|| / ||b||, residual in double-double"""
+    r = g.residual(b, x)
+    return np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0))
+
+
+@pytest.mark.parametrize("which", ["SM", "LU"])
+def test_mixed_gauss1d_reaches_target(which):
+    _case(f"mixed gauss1d {which}")
+    cfg = I.config_inputs("gauss1d_4096")
+    g = HODLR_Matrix.from_kernel(cfg["points"], "gaussian", 0.1, 1e-2, sizeThreshold=64)
+    g.set_LRTolerance(1e-10)
+    solve = g.recSM_Solve if which == "SM" else g.recLU_Solve
+    x64 = solve(cfg["rhs"])
+    g.set_precision("mixed")
+    g.set_refinement(1e-12, 30)
+    xm = solve(cfg["rhs"])
+    st = g.stats()
+    assert st["last_residual"] <= 1e-12, st
+    assert colres(g, xm, cfg["rhs"]) <= 2e-12  # measured independently of the loop
+    assert np.linalg.norm(xm - x64) / np.linalg.norm(x64) < 1e-9
+    assert 1 <= st["refine_iterations"] <= 30
+
+
+@pytest.mark.parametrize("prec,budget", [("fp64_refined", 6), ("mixed", 40)])
+def test_refined_gauss2d_reaches_1e10(prec, budget):
+    """2D Gaussian, l=0.05, shift 1e-4 (condition number ~1e7): the plain FP64
+    recSM solution stops near 1e-7; flexible GMRES refinement against A_H
+    brings it to 1e-10 with either preconditioner."""
+    _case(f"{prec} gauss2d 16384")
+    n = 16384
+    pts = I.points_2d_kd(n, 5)
+    b = I.rhs_normal(n, 8, 6)
+    g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+    g.set_LRTolerance(1e-10)
+    x64 = g.recSM_Solve(b)
+    r64 = colres(g, x64, b)
+    g.set_precision(prec)
+    g.set_refinement(1e-10, budget)
+    xm = g.recSM_Solve(b)
+    st = g.stats()
+    print(f"fp64 residual {r64:.2e}; {prec} residual {st['last_residual']:.2e} after "
+          f"{st['refine_iterations']} steps; rel. diff to fp64 "
+          f"{np.linalg.norm(xm - x64) / np.linalg.norm(x64):.2e}", flush=True)
+    assert r64 > 1e-9  # the unrefined solve does not reach the target here
+    assert st["last_residual"] <= 1e-10, st
+    assert colres(g, xm, b) <= 1.5e-10
+    assert st["refine_iterations"] <= budget
+
+
+def test_mixed_settings_switch_back():
+    _case("mixed switch back")
+    n = 2048
+    pts = I.points_1d_sorted(n, 3)
+    b = I.rhs_uniform(n, 2, 4)
+    g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.1, 1e-2, sizeThreshold=64)
+    g.set_LRTolerance(1e-10)
+    x1 = g.recSM_Solve(b)
+    g.set_precision("mixed")
+    g.recSM_Solve(b)
+    g.set_precision("fp64")
+    x2 = g.recSM_Solve(b)
+    assert np.array_equal(x1, x2)
