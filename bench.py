@@ -49,7 +49,21 @@ def parse():
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
+    p.add_argument("--copy-kernel", action="store_true",
+                   help="only run a device-to-device copy of 1 GiB five times (calibrates the "
+                        "HBM byte counters of rocprofv3 --pmc) and exit")
     return p.parse_args()
+
+
+def copy_kernel():
+    import torch
+    x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")  # 1 GiB
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8,
+                      "launches": 5}), flush=True)
 
 
 def workload(n, nrhs, seed=12345):
@@ -110,6 +124,9 @@ def cpu_baseline(sizes, nrhs):
 
 def main():
     a = parse()
+    if a.copy_kernel:
+        copy_kernel()
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "one GPU only: this build runs the whole system on a single "
                                    "MI355X; --gpus must be 1"}))
