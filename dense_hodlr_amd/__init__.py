@@ -3,6 +3,6 @@ amiraa127/Dense_HODLR (kernel/dense constructors, partial-pivot ACA or SVD
 compression, recSM/recLU solves, log-determinant), built as hand-written HIP
 kernels for gfx950 behind the C ABI in include/dense_hodlr.h."""
 from ._lib import HodlrError, lib
-from .hodlr_matrix import HODLR_Matrix
+from .hodlr_matrix import HODLR_Batch, HODLR_Matrix
 
-__all__ = ["HODLR_Matrix", "HodlrError", "lib"]
+__all__ = ["HODLR_Matrix", "HODLR_Batch", "HodlrError", "lib"]

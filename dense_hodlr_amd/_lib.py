@@ -34,6 +34,7 @@ EXPORTS = [
     "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
     "hodlr_matvec", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
     "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
+    "hodlr_create_batch", "hodlr_batch_count", "hodlr_batch_offsets", "hodlr_logabsdet_batch",
 ]
 
 
@@ -120,6 +121,10 @@ def lib():
         "hodlr_get_factor": (C.c_int, [h, C.c_int, C.c_int, dp]),
         "hodlr_last_error": (C.c_char_p, [h]),
         "hodlr_version": (C.c_char_p, []),
+        "hodlr_create_batch": (C.c_int, [hp, C.c_int, ip, C.POINTER(KernelDesc), C.c_int]),
+        "hodlr_batch_count": (C.c_int, [h]),
+        "hodlr_batch_offsets": (C.c_int, [h, ip]),
+        "hodlr_logabsdet_batch": (C.c_int, [h, dp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

@@ -126,7 +126,7 @@ __device__ PivRed reduce_slots(const AcaLevels& L, const AcaBlock& b, int s0, in
 __device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32_t* colbits,
                              int min_pivot, PivRed* sh) {
   const PivRed r = reduce_slots(L, b, 0, b.row_nwg, sh);
-  if (r.v <= (double)min_pivot) return -1;
+  if (r.v <= (double)min_pivot || r.v < HODLR_PIVOT_FLOOR) return -1;
   if (!bit_test(colbits, r.i)) return r.i;
   return r.first >= 0 ? r.first : -2;
 }
@@ -291,7 +291,8 @@ __global__ void __launch_bounds__(TPB)
       Vp[(size_t)jj[q] + (size_t)k * ld] = r;
       const double ar = fabs(r);
       argmax_merge(p.v, p.i, ar, jj[q]);
-      if (!bit_test(colbits, jj[q]) && ar > (double)min_pivot && (p.first < 0 || jj[q] < p.first))
+      if (!bit_test(colbits, jj[q]) && ar > (double)min_pivot && ar >= HODLR_PIVOT_FLOOR &&
+          (p.first < 0 || jj[q] < p.first))
         p.first = jj[q];
     }
   }
@@ -366,7 +367,8 @@ __global__ void __launch_bounds__(TPB)
       Wp[(size_t)ii[q] + (size_t)k * ld] = pivot * c;  // :230
       const double ac = fabs(c);
       argmax_merge(p.v, p.i, ac, ii[q]);
-      if (!bit_test(rowbits, ii[q]) && ac > (double)min_pivot && (p.first < 0 || ii[q] < p.first))
+      if (!bit_test(rowbits, ii[q]) && ac > (double)min_pivot && ac >= HODLR_PIVOT_FLOOR &&
+          (p.first < 0 || ii[q] < p.first))
         p.first = ii[q];
     }
   }

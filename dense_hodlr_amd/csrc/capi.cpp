@@ -70,23 +70,15 @@ int make_handle(hodlr_handle* h, std::vector<hodlr::InstanceSpec>&& specs) {
   return HODLR_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int leaf) {
-  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
-  *h = nullptr;
-  if (n <= 0 || leaf <= 0 || !k || !k->points)
-    return fail(nullptr, HODLR_ERR_ARG, "n, leaf must be positive and points non-null");
-  if (k->dim < 1 || k->dim > 3) return fail(nullptr, HODLR_ERR_ARG, "dim must be 1, 2 or 3");
-  if (k->family < 0 || k->family > HODLR_KERNEL_EXPONENTIAL)
-    return fail(nullptr, HODLR_ERR_ARG, "unknown kernel family");
-  if (!(k->length_scale > 0.0)) return fail(nullptr, HODLR_ERR_ARG, "length_scale must be > 0");
+/* Validates a typed kernel descriptor and fills one instance; returns the
+ * error message, empty on success. */
+std::string kernel_spec(int n, const hodlr_kernel_desc* k, int leaf, hodlr::InstanceSpec& s) {
+  if (n <= 0 || leaf <= 0 || !k || !k->points) return "n, leaf must be positive and points non-null";
+  if (k->dim < 1 || k->dim > 3) return "dim must be 1, 2 or 3";
+  if (k->family < 0 || k->family > HODLR_KERNEL_EXPONENTIAL) return "unknown kernel family";
+  if (!(k->length_scale > 0.0)) return "length_scale must be > 0";
   if (k->diag_mode != HODLR_DIAG_ADD_SHIFT && k->diag_mode != HODLR_DIAG_REPLACE_VALUE)
-    return fail(nullptr, HODLR_ERR_ARG, "unknown diag_mode");
-  std::vector<hodlr::InstanceSpec> specs(1);
-  hodlr::InstanceSpec& s = specs[0];
+    return "unknown diag_mode";
   s.n = n;
   s.leaf = leaf;
   s.kind = 0;
@@ -100,7 +92,58 @@ int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int 
   }
   hodlr_kernel_params_init(&s.kp, k->family, k->dim, k->length_scale, k->diag, k->diag_mode,
                            nullptr, nullptr, nullptr);
+  return std::string();
+}
+
+}  // namespace
+
+extern "C" {
+
+int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int leaf) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  std::vector<hodlr::InstanceSpec> specs(1);
+  const std::string err = kernel_spec(n, k, leaf, specs[0]);
+  if (!err.empty()) return fail(nullptr, HODLR_ERR_ARG, err);
   return make_handle(h, std::move(specs));
+}
+
+int hodlr_create_batch(hodlr_handle* h, int count, const int* n, const hodlr_kernel_desc* kernels,
+                       int leaf) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (count <= 0 || !n || !kernels) return fail(nullptr, HODLR_ERR_ARG, "bad arguments");
+  std::vector<hodlr::InstanceSpec> specs(count);
+  long long tot = 0;
+  for (int q = 0; q < count; q++) {
+    const std::string err = kernel_spec(n[q], &kernels[q], leaf, specs[q]);
+    if (!err.empty()) return fail(nullptr, HODLR_ERR_ARG, "instance " + std::to_string(q) + ": " + err);
+    tot += n[q];
+  }
+  if (tot > 0x7fffffffLL) return fail(nullptr, HODLR_ERR_ARG, "total size exceeds 2^31 - 1 rows");
+  return make_handle(h, std::move(specs));
+}
+
+int hodlr_batch_count(hodlr_handle h) {
+  if (!h) return HODLR_ERR_ARG;
+  return h->eng->num_instances();
+}
+
+int hodlr_batch_offsets(hodlr_handle h, int* row0) {
+  return guard(h, [&] {
+    if (!row0) throw std::invalid_argument("null output");
+    const auto& r = h->eng->instance_rows();
+    for (size_t q = 0; q < r.size(); q++) row0[q] = r[q];
+    row0[r.size()] = h->eng->rows();
+  });
+}
+
+int hodlr_logabsdet_batch(hodlr_handle h, double* out) {
+  return guard(h, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    std::vector<double> v = h->eng->logabsdet_instances();
+    for (size_t q = 0; q < v.size(); q++) out[q] = v[q];
+  });
 }
 
 int hodlr_create_kernel_callback(hodlr_handle* h, int n, double (*fn)(int, int, void*), void* data,

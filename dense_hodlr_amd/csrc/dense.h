@@ -130,6 +130,13 @@ void launch_balance(const BalanceDesc* d, int count, int kmax, hipStream_t s);
 /* out[c] = sum_i X[i + c*ld]^2 for c < ncols (one workgroup per column, fixed
  * reduction order) */
 void launch_colnorm2(const double* X, int n, int ld, int ncols, double* out, hipStream_t s);
+/* Per instance segment: out[s + c*nseg] = sum of squares of rows
+ * [row0[s], row0[s+1]) of column c; segscale multiplies those rows by
+ * sc[s + c*nseg].  row0 has nseg + 1 entries (device). */
+void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, double* out,
+                     hipStream_t s);
+void launch_segscale(double* X, int ld, const int* row0, int nseg, int ncols, const double* sc,
+                     hipStream_t s);
 /* X += alpha * D (FP32 correction) over an n x ncols block */
 void launch_add_f32(double* X, int ldx, const float* D, int ldd, int n, int ncols, hipStream_t s);
 /* Batched column operations of the block GMRES: dots of w with nvec basis

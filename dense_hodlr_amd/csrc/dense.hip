@@ -157,6 +157,9 @@ __global__ void __launch_bounds__(TPB) lu_panel_kernel(const LuDesc* __restrict_
           v = sv[w];
           ix = si[w];
         }
+      // a column of NaNs has no maximum: keep row c (never an index >= n,
+      // which would swap with memory past the matrix)
+      if (ix >= n) ix = c;
       piv_sh = ix;
       m.piv[c] = ix;
     }
@@ -291,6 +294,33 @@ __global__ void __launch_bounds__(TPB) balance_kernel(const BalanceDesc* __restr
   float* v32 = b.V32 + (size_t)c * b.ldv32;
   for (int i = threadIdx.x; i < b.mu; i += TPB) u32[i] = (float)(u[i] * s);
   for (int i = threadIdx.x; i < b.mv; i += TPB) v32[i] = (float)(v[i] * is);
+}
+
+/* squared norms of rows [row0[s], row0[s+1]) of column c: out[s + c * nseg] */
+__global__ void __launch_bounds__(TPB)
+    segnorm2_kernel(const double* __restrict__ X, int ld, const int* __restrict__ row0, int nseg,
+                    double* __restrict__ out) {
+  __shared__ double sh[TPB];
+  const int sg = blockIdx.y, c = blockIdx.x;
+  const double* x = X + (size_t)c * ld;
+  double acc = 0.0;
+  for (int i = row0[sg] + threadIdx.x; i < row0[sg + 1]; i += TPB) acc += x[i] * x[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = TPB / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[sg + (size_t)c * nseg] = sh[0];
+}
+
+__global__ void __launch_bounds__(TPB)
+    segscale_kernel(double* __restrict__ X, int ld, const int* __restrict__ row0, int nseg,
+                    const double* __restrict__ sc) {
+  const int sg = blockIdx.y, c = blockIdx.x;
+  const double f = sc[sg + (size_t)c * nseg];
+  double* x = X + (size_t)c * ld;
+  for (int i = row0[sg] + threadIdx.x; i < row0[sg + 1]; i += TPB) x[i] *= f;
 }
 
 __global__ void __launch_bounds__(TPB)
@@ -472,6 +502,22 @@ void launch_nonfinite(const T* x, size_t n, unsigned long long* out, hipStream_t
 }
 template void launch_nonfinite<float>(const float*, size_t, unsigned long long*, hipStream_t);
 template void launch_nonfinite<double>(const double*, size_t, unsigned long long*, hipStream_t);
+
+void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, double* out,
+                     hipStream_t s) {
+  if (ncols <= 0 || nseg <= 0) return;
+  dim3 g(ncols, nseg);
+  segnorm2_kernel<<<g, TPB, 0, s>>>(X, ld, row0, nseg, out);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_segscale(double* X, int ld, const int* row0, int nseg, int ncols, const double* sc,
+                     hipStream_t s) {
+  if (ncols <= 0 || nseg <= 0) return;
+  dim3 g(ncols, nseg);
+  segscale_kernel<<<g, TPB, 0, s>>>(X, ld, row0, nseg, sc);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
 
 void launch_colnorm2(const double* X, int n, int ld, int ncols, double* out, hipStream_t s) {
   if (ncols <= 0) return;

@@ -1306,10 +1306,27 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
   double* V = basis_.p;
   double* dcoef = nbuf_.p;            // (m + 1) x nrhs coefficients
   double* dnorm = nbuf_.p + (size_t)(m + 1) * nrhs;  // nrhs values
+  // Norms per instance segment and column: every instance of a batch must
+  // reach the target on its own (rows of instance q: inst_row0_[q] ..).
+  const int nseg = (int)specs_.size();
+  if (seg_rows_.n < (size_t)nseg + 1) {
+    std::vector<int> r0(inst_row0_);
+    r0.push_back(ntot_);
+    seg_rows_.alloc(r0.size());
+    HODLR_HIP_CHECK(hipMemcpy(seg_rows_.p, r0.data(), r0.size() * sizeof(int), hipMemcpyHostToDevice));
+  }
+  grow(segbuf_, (size_t)nseg * nrhs);
   auto colnorms = [&](const double* X, std::vector<double>& out) {
     launch_colnorm2(X, ntot_, ld, nrhs, dnorm, stream_);
     out.resize(nrhs);
     HODLR_HIP_CHECK(hipMemcpyAsync(out.data(), dnorm, nrhs * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    for (auto& v : out) v = std::sqrt(v);
+  };
+  auto segnorms = [&](const double* X, std::vector<double>& out) {
+    launch_segnorm2(X, ld, seg_rows_.p, nseg, nrhs, segbuf_.p, stream_);
+    out.resize((size_t)nseg * nrhs);
+    HODLR_HIP_CHECK(hipMemcpyAsync(out.data(), segbuf_.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
     for (auto& v : out) v = std::sqrt(v);
   };
@@ -1330,13 +1347,33 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
       solve_fp64_inplace(which, out, nrhs);
     }
   };
-  std::vector<double> bn, rn, hn, coef((size_t)(m + 1) * nrhs);
+  std::vector<double> bn, rs, rn(nrhs), hn, coef((size_t)(m + 1) * nrhs);
   HODLR_HIP_CHECK(hipMemsetAsync(xw_.p, 0, N * nrhs * sizeof(double), stream_));
   HODLR_HIP_CHECK(hipMemcpy2DAsync(rbuf_.p, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
                                    N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
-  colnorms(rbuf_.p, bn);
+  segnorms(rbuf_.p, bn);
   for (auto& v : bn)
     if (!(v > 0)) v = 1.0;
+  // A batch is solved with every instance's right-hand side scaled to unit
+  // norm (the Krylov minimization is over all instances at once) and the
+  // solution scaled back at the end.
+  const double* Beff = dB;
+  int ldbeff = ldb;
+  const std::vector<double> bscale = bn;
+  if (nseg > 1) {
+    grow(bscaled_, N * nrhs);
+    HODLR_HIP_CHECK(hipMemcpyAsync(bscaled_.p, rbuf_.p, N * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    std::vector<double> inv(bn.size());
+    for (size_t e = 0; e < bn.size(); e++) inv[e] = 1.0 / bn[e];
+    upload(inv, segbuf_.p);
+    launch_segscale(bscaled_.p, ld, seg_rows_.p, nseg, nrhs, segbuf_.p, stream_);
+    HODLR_HIP_CHECK(hipMemcpyAsync(rbuf_.p, bscaled_.p, N * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
+    Beff = bscaled_.p;
+    ldbeff = ld;
+    segnorms(rbuf_.p, bn);
+    for (auto& v : bn)
+      if (!(v > 0)) v = 1.0;
+  }
   // Hessenberg, Givens rotations and right-hand side per column (host)
   std::vector<double> H((size_t)(m + 1) * m * nrhs), cs((size_t)m * nrhs), sn((size_t)m * nrhs),
       g((size_t)(m + 1) * nrhs);
@@ -1344,20 +1381,27 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
   int total = 0;
   double res = 0.0;
   for (;;) {
-    colnorms(rbuf_.p, rn);
+    segnorms(rbuf_.p, rs);
     res = 0.0;
+    std::vector<char> active(nrhs, 0);
     for (int c = 0; c < nrhs; c++) {
-      const double rc = rn[c] / bn[c];
-      if (!(rc <= res)) res = rc;
+      double s2 = 0.0;
+      for (int q = 0; q < nseg; q++) {
+        const size_t e = (size_t)q + (size_t)c * nseg;
+        const double rc = rs[e] / bn[e];
+        if (!(rc <= res)) res = rc;
+        if (rc > refine_target_) active[c] = 1;
+        s2 += rs[e] * rs[e];
+      }
+      rn[c] = std::sqrt(s2);
     }
     if (trace_) std::fprintf(stderr, "[gmres] steps %d true residual %.3e\n", total, res);
     if (!std::isfinite(res)) throw std::runtime_error("refinement produced a non-finite residual");
     if (res <= refine_target_ || total >= refine_max_) break;
     // V_0 = r / ||r||; converged columns get a zero basis and stay put
     std::vector<double> inv(nrhs);
-    std::vector<char> active(nrhs);
     for (int c = 0; c < nrhs; c++) {
-      active[c] = rn[c] / bn[c] > refine_target_ && rn[c] > 0;
+      active[c] = active[c] && rn[c] > 0;
       inv[c] = active[c] ? 1.0 / rn[c] : 0.0;
       g[(size_t)c * (m + 1)] = active[c] ? rn[c] : 0.0;
       for (int i = 1; i <= m; i++) g[(size_t)c * (m + 1) + i] = 0.0;
@@ -1416,8 +1460,14 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
       launch_col_scale(Vn, ld, ntot_, nrhs, dnorm, stream_);
       total++;
       jend = j + 1;
+      // Arnoldi estimate of the column residual, relative to the smallest
+      // instance right-hand side of the column
       double est = 0.0;
-      for (int c = 0; c < nrhs; c++) est = std::max(est, std::fabs(g[(size_t)c * (m + 1) + j + 1]) / bn[c]);
+      for (int c = 0; c < nrhs; c++) {
+        double bmin = bn[(size_t)c * nseg];
+        for (int q = 1; q < nseg; q++) bmin = std::min(bmin, bn[(size_t)q + (size_t)c * nseg]);
+        est = std::max(est, std::fabs(g[(size_t)c * (m + 1) + j + 1]) / bmin);
+      }
       if (trace_) std::fprintf(stderr, "[gmres] step %d estimate %.3e\n", total, est);
       if (est <= 0.5 * refine_target_ || total >= refine_max_) break;
     }
@@ -1437,7 +1487,11 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     std::vector<double> yv(coef.begin(), coef.begin() + (size_t)jend * nrhs);
     upload(yv, dcoef);
     launch_multi_axpy(zbasis_.p, vstride, jend, dcoef, xw_.p, ntot_, ld, nrhs, 1.0, 0, stream_);
-    residual_dd(dB, ldb, xw_.p, ld, nrhs, rbuf_.p, ld);
+    residual_dd(Beff, ldbeff, xw_.p, ld, nrhs, rbuf_.p, ld);
+  }
+  if (nseg > 1) {  // undo the right-hand side scaling
+    upload(bscale, segbuf_.p);
+    launch_segscale(xw_.p, ld, seg_rows_.p, nseg, nrhs, segbuf_.p, stream_);
   }
   stats_.refine_iterations = total;
   stats_.last_residual = res;
@@ -1454,24 +1508,41 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
 
 /* calcDeterminant (HODLR_Matrix.cpp:1671-1697): sum of log|diag LU| over the
  * leaves and the recSM capacitance matrices (perturbI.cpp:61-73). */
-double Engine::logabsdet(double* det_prod) {
-  factorize(0);
+/* Per factor (leaves, then nodes) log|det| and product of the LU diagonal of
+ * the FP64 recSM factorization; inst receives the owning instance. */
+void Engine::logdet_terms(std::vector<double>& hl, std::vector<double>& hp, std::vector<int>& inst) {
+  factorize_fp64(0);
   Factorization& F = fac_[0];
   std::vector<LuDesc> ld;
-  for (auto& lf : leaves_) ld.push_back({Lfac_.p + lf.mat_off, nullptr, lf.s, lf.s});
+  inst.clear();
+  for (auto& lf : leaves_) {
+    ld.push_back({Lfac_.p + lf.mat_off, nullptr, lf.s, lf.s});
+    inst.push_back(lf.inst);
+  }
   for (int i = 0; i < (int)nodes_.size(); i++) {
     const int K = nodes_[i].kt + nodes_[i].kb;
     ld.push_back({F.C.p + F.c_off[i], nullptr, K, K});
+    inst.push_back(nodes_[i].inst);
   }
   DBuf<double> lg, pr;
   lg.alloc(ld.size());
   pr.alloc(ld.size());
   launch_logdet(arena_.push(ld, stream_), (int)ld.size(), lg.p, pr.p, stream_);
-  std::vector<double> hl(ld.size()), hp(ld.size());
+  hl.assign(ld.size(), 0.0);
+  hp.assign(ld.size(), 0.0);
   HODLR_HIP_CHECK(hipMemcpyAsync(hl.data(), lg.p, hl.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HODLR_HIP_CHECK(hipMemcpyAsync(hp.data(), pr.p, hp.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
   HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
   arena_.reset();
+}
+
+/* logAbsDeterminant (HODLR_Matrix.cpp:1666-1697): sum over leaf LUs and node
+ * capacitance LUs (Sylvester's identity, perturbI.cpp:61-83).  Always from
+ * the FP64 factorization. */
+double Engine::logabsdet(double* det_prod) {
+  std::vector<double> hl, hp;
+  std::vector<int> inst;
+  logdet_terms(hl, hp, inst);
   double s = 0, p = 1;
   for (size_t i = 0; i < hl.size(); i++) {
     s += hl[i];
@@ -1479,6 +1550,15 @@ double Engine::logabsdet(double* det_prod) {
   }
   if (det_prod) *det_prod = p;
   return s;
+}
+
+std::vector<double> Engine::logabsdet_instances() {
+  std::vector<double> hl, hp;
+  std::vector<int> inst;
+  logdet_terms(hl, hp, inst);
+  std::vector<double> out(specs_.size(), 0.0);
+  for (size_t i = 0; i < hl.size(); i++) out[inst[i]] += hl[i];
+  return out;
 }
 
 /* Y = A_H X: leaves plus, per node, U_t (V_t^T X_bot) and U_b (V_b^T X_top). */

@@ -164,6 +164,9 @@ class Engine {
 
   // inspection
   int rows() const { return ntot_; }
+  int num_instances() const { return (int)specs_.size(); }
+  const std::vector<int>& instance_rows() const { return inst_row0_; }
+  std::vector<double> logabsdet_instances();
   int num_levels() const { return nlev_; }
   const std::vector<TreeNodeH>& nodes() const { return nodes_; }
   const std::vector<LeafH>& leaves() const { return leaves_; }
@@ -196,6 +199,7 @@ class Engine {
   void grow(DBuf<double>& b, size_t doubles);
   void lu_batch(std::vector<LuDesc>& descs);
   void factorize_fp64(int which);
+  void logdet_terms(std::vector<double>& hl, std::vector<double>& hp, std::vector<int>& inst);
   void factorize_mixed(int which);
   void solve_fp64_inplace(int which, double* X, int nrhs);
   void solve_mixed_once(int which, float* X, int nrhs);
@@ -249,6 +253,8 @@ class Engine {
   DBuf<double> ws_, tmp_, tmp2_;
   DBuf<double> xw_;
   DBuf<double> ddy_, ddpart_, ddt_;  // double-double residual: Y, partials, projections
+  DBuf<int> seg_rows_;               // instance row offsets (+ total) on the device
+  DBuf<double> segbuf_, bscaled_;    // per-instance norms / scales, scaled batch RHS
   Stats stats_;
 };
 

@@ -28,6 +28,16 @@
 #define HODLR_NO_CONTRACT
 #endif
 
+/* Smallest magnitude an ACA pivot candidate may have, 2^-511: its square and
+ * the square of its reciprocal are normal FP64 numbers.  The reference
+ * (lowRank.cpp:170-227, chooseNextRowCol :291-298) accepts any value above
+ * minPivot (an int, normally 0), so on kernels whose far entries underflow
+ * (a squared exponential with l = 0.01 on the unit interval) it divides by
+ * subnormal pivots and returns non-finite factors.  Candidates below this
+ * floor count as zero here, in the oracle and in the GPU kernels alike;
+ * inputs whose candidates all exceed it behave exactly as in the reference. */
+#define HODLR_PIVOT_FLOOR 0x1p-511
+
 /* Kernel families.  r is the Euclidean distance, l the length scale. */
 enum {
   HODLR_KF_GAUSSIAN = 0,    /* exp(-r^2 / l^2)                      */

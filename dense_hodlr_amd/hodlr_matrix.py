@@ -198,4 +198,53 @@ class HODLR_Matrix:
         return out
 
 
-__all__ = ["HODLR_Matrix", "HodlrError"]
+class HODLR_Batch(HODLR_Matrix):
+    """Many independent kernel systems in one handle (stacked rows).
+
+    points: list of arrays (n_q x dim); kernels, length_scales, diags: one
+    value per instance or a scalar for all.  Solves take and return the
+    stacked right-hand sides (sum n_q rows); split() cuts a stacked array into
+    the instances' parts.  No reference counterpart: a batch of the
+    reference's HODLR_Matrix objects processed together.
+    """
+
+    def __init__(self, points, kernel="se", length_scales=1.0, diags=0.0, sizeThreshold=30,
+                 diag_mode="add"):
+        count = len(points)
+        pts = [np.ascontiguousarray(np.asarray(p, dtype=np.float64)) for p in points]
+        pts = [p.reshape(-1, 1) if p.ndim == 1 else p for p in pts]
+
+        def per(v):
+            return list(v) if np.ndim(v) else [v] * count
+
+        fam, ls, dg = per(kernel), per(length_scales), per(diags)
+        descs = (_lib.KernelDesc * count)()
+        for q, p in enumerate(pts):
+            descs[q] = _lib.KernelDesc(_lib.KERNELS[fam[q]], p.shape[1], _dptr(p), float(ls[q]),
+                                       float(dg[q]), {"add": 0, "replace": 1}[diag_mode])
+        ns = np.array([p.shape[0] for p in pts], dtype=np.int32)
+        h = C.c_void_p()
+        check(lib().hodlr_create_batch(C.byref(h), count, ns.ctypes.data_as(C.POINTER(C.c_int)),
+                                       descs, int(sizeThreshold)))
+        super().__init__(_handle=h, _n=int(ns.sum()))
+        self._count = count
+
+    def count(self) -> int:
+        return lib().hodlr_batch_count(self._h)
+
+    def offsets(self):
+        out = np.zeros(self._count + 1, dtype=np.int32)
+        check(lib().hodlr_batch_offsets(self._h, out.ctypes.data_as(C.POINTER(C.c_int))), self._h)
+        return out
+
+    def split(self, X):
+        o = self.offsets()
+        return [X[o[q]:o[q + 1]] for q in range(self._count)]
+
+    def logAbsDeterminants(self):
+        out = np.zeros(self._count)
+        check(lib().hodlr_logabsdet_batch(self._h, _dptr(out)), self._h)
+        return out
+
+
+__all__ = ["HODLR_Matrix", "HODLR_Batch", "HodlrError"]

@@ -196,4 +196,28 @@ def config_inputs(name: str, seed: int = 12345):
         n = 262144
         return dict(n=n, points=points_2d_kd(n, seed), family="gaussian", length_scale=0.05,
                     diag=1e-4, leaf=256, tol=1e-10, rhs=rhs_normal(n, 64, seed + 1))
+    if name == "gp_batch_512":
+        return batch_inputs(512, 1024, 16384, seed)
     raise KeyError(name)
+
+
+def batch_inputs(count: int, nmin: int, nmax: int, seed: int = 12345):
+    """Independent Gaussian-process covariance systems: sizes log-uniform in
+    [nmin, nmax], 1D sorted uniform points, squared-exponential kernel
+    exp(-r^2 / (2 l^2)) with l uniform in [0.01, 0.2], noise variance 1e-2 on
+    the diagonal, one N(0,1) right-hand side each, leaf 64, ACA tolerance
+    1e-10.  One generator draws, per instance in order: the size, the length
+    scale, then (from seeds derived per instance) points and right-hand side."""
+    g = SplitMix64(seed)
+    ns, ls, pts, rhs = [], [], [], []
+    lo, hi = np.log(nmin), np.log(nmax)
+    for q in range(count):
+        n = int(np.floor(np.exp(lo + (hi - lo) * g.uniform(1)[0])))
+        n = min(max(n, nmin), nmax)
+        ell = float(0.01 + 0.19 * g.uniform(1)[0])
+        ns.append(n)
+        ls.append(ell)
+        pts.append(points_1d_sorted(n, seed * 7919 + 2 * q + 1))
+        rhs.append(rhs_normal(n, 1, seed * 7919 + 2 * q + 2))
+    return dict(count=count, n=ns, length_scale=ls, points=pts, family="se", diag=1e-2, leaf=64,
+                tol=1e-10, rhs=rhs)

@@ -177,6 +177,28 @@ int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, 
 int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const double* X, int ldx,
                    double* R, int ldr);
 
+/* ---- batched instances (no reference counterpart) ----------------------- */
+
+/* count independent kernel systems in one handle, for many small or uneven
+ * problems (e.g. Gaussian-process covariance systems): instance q has n[q]
+ * points and its own kernel descriptor; all share the leaf size.  They form
+ * one block-diagonal system of order sum n[q] whose rows are the instances'
+ * rows in order, and every call on the handle (compress, factorize, solves,
+ * refinement, matvec, residual) processes all instances level-synchronously:
+ * one launch per kernel and tree level serves the whole batch.  Right-hand
+ * sides and solutions are the stacked instance vectors; with refinement each
+ * instance reaches the target on its own. */
+int hodlr_create_batch(hodlr_handle* h, int count, const int* n, const hodlr_kernel_desc* kernels,
+                       int leaf);
+/* Number of instances (1 for the single-matrix constructors). */
+int hodlr_batch_count(hodlr_handle h);
+/* First row of each instance in the stacked layout, plus the total
+ * (count + 1 entries). */
+int hodlr_batch_offsets(hodlr_handle h, int* row0);
+/* log|det| of each instance (count entries), as hodlr_logabsdet computes it
+ * for one matrix. */
+int hodlr_logabsdet_batch(hodlr_handle h, double* out);
+
 /* ---- inspection -------------------------------------------------------- */
 
 int hodlr_get_stats(hodlr_handle h, hodlr_stats* out);

@@ -187,8 +187,9 @@ void set_rows(Mat& A, int r0, const Mat& B) {
 int choose_next_row_col(const std::vector<char>& chosen, const std::vector<double>& absv,
                         int minPivot) {
   // lowRank.cpp:291-298: the FIRST unchosen index whose |value| exceeds minPivot
+  // (and the pivot floor, HODLR_PIVOT_FLOOR)
   for (size_t idx = 0; idx < absv.size(); idx++)
-    if (!chosen[idx] && absv[idx] > minPivot) return (int)idx;
+    if (!chosen[idx] && absv[idx] > minPivot && absv[idx] >= HODLR_PIVOT_FLOOR) return (int)idx;
   return -1;
 }
 
@@ -243,7 +244,7 @@ double aca_partial_pivot(const Src& A, int i0, int j0, int m, int n, double tol,
       absr[j] = std::fabs(rrow[j]);
       if (absr[j] > maxValue) { maxValue = absr[j]; maxInd = j; }
     }
-    if (maxValue <= minPivot) {  // :187-195: jump to the first unchosen row
+    if (maxValue <= minPivot || maxValue < HODLR_PIVOT_FLOOR) {  // :187-195: jump to the first unchosen row
       auto it = std::find(chosenRows.begin(), chosenRows.end(), 0);
       if (it == chosenRows.end()) break;
       currRow = (int)(it - chosenRows.begin());

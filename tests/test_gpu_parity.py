@@ -203,3 +203,26 @@ def test_settings_invalidate_cache(oracle):
     x2 = g.recSM_Solve(b)
     assert g.nodes()[:, 4:].sum() > r1
     assert rel(x1, x2) > 1e-12
+
+
+def test_underflowing_kernel_matches_oracle(oracle):
+    """Squared exponential with l = 0.0103 on 1321 sorted points: the far
+    entries of the top blocks are subnormal or zero.  Pivot candidates below
+    the floor 2^-511 count as zero on both sides, so the factors stay finite
+    and bit-identical to the oracle's."""
+    _case("se l=0.0103 n=1321 subnormal far field")
+    cfg = I.batch_inputs(24, 1024, 8192, seed=5)
+    q = 19
+    n, ell = cfg["n"][q], cfg["length_scale"][q]
+    assert n == 1321 and abs(ell - 0.0103) < 1e-4
+    g, o = _pair(oracle, n, cfg["points"][q], "se", ell, 1e-2, 64, 1e-10)
+    g.storeLRinTree()
+    o.compress()
+    assert np.array_equal(g.nodes(), o.nodes())
+    for idx in range(len(o.nodes())):
+        for which in range(4):
+            a, b = g.factor(idx, which), o.factor(idx, which)
+            assert np.all(np.isfinite(a)) and np.array_equal(a, b), (idx, which)
+    b = cfg["rhs"][q]
+    xg, xo = g.recSM_Solve(b), o.recSM_Solve(b)
+    assert np.all(np.isfinite(xg)) and rel(xg, xo) < 1e-10

@@ -305,3 +305,32 @@ def test_gauss1d_workload_against_dense_lu(oracle):
     # backward error w.r.t. the exact kernel matrix sits at the ACA tolerance
     bwd = np.linalg.norm(A @ x - cfg["rhs"]) / (np.linalg.norm(A, 2) * np.linalg.norm(x))
     assert bwd < 1e-10
+
+
+def test_aca_subnormal_pivots_count_as_zero(oracle):
+    """Candidates below the pivot floor 2^-511 are treated as zero (the
+    reference divides by them and returns non-finite factors): an all-tiny
+    block compresses to the zero block, and a block with one normal entry
+    among subnormal ones gives that single term."""
+    tiny = np.full((12, 9), 4e-310)
+    W, V, r, _, _ = oracle.aca(tiny, 1e-10)
+    assert np.all(np.isfinite(W)) and np.all(np.isfinite(V))
+    assert np.max(np.abs(W @ V.T)) == 0.0
+    # rows 0-2 are tiny: jump to row 3, pivot (3, 4); the column gives row 7
+    # as the next row (row 3 is taken, the others are below the floor), so
+    # the term is kept
+    A = tiny.copy()
+    A[3, 4] = 0.5
+    A[7, 4] = 0.25
+    W, V, r, rows, cols = oracle.aca(A, 1e-10)
+    assert np.all(np.isfinite(W)) and np.all(np.isfinite(V))
+    # the residual of row 7 is then below the floor everywhere: the remaining
+    # rows are jumped over and the rank stays 1
+    assert r == 1 and rows[0] == 3 and cols[0] == 4
+    S = W @ V.T
+    assert abs(S[3, 4] - 0.5) < 1e-15 and abs(S[7, 4] - 0.25) < 1e-15
+    # a block whose entries all exceed the floor is unaffected (same result
+    # as with the floor far below them)
+    B = np.outer(np.linspace(1e-150, 2e-150, 7), np.linspace(1.0, 3.0, 5))
+    W, V, r, _, _ = oracle.aca(B, 1e-10)
+    assert r == 1 and np.allclose(W @ V.T, B, rtol=1e-14, atol=0)
