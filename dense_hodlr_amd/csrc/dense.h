@@ -31,13 +31,16 @@ struct SolveDesc {
   int pad_;
 };
 
-/* Dense block of entries: out[i + j*ldo] = K(i0 + i, j0 + j) of instance inst. */
+/* Dense block of entries: out[i + j*ldo] = K(i0 + i, j0 + j) of instance
+ * inst (i < m, j < n), or its transpose out[j + i*ldo] when trans = 1. */
 struct AssembleDesc {
   double* out;
   int inst;
   int i0, j0;
   int m, n;
   int ldo;
+  int trans;
+  int pad_;
 };
 
 /* Copy of an m x n column-major block. */

