@@ -7,12 +7,15 @@ Points and right-hand sides are resident in device memory before the timed
 region.
 
 One step = one hodlr_solve_device call on a fresh handle: compression (ACA
-with fused kernel assembly), recSM factorization, solve, and flexible-GMRES
-refinement against the FP64 HODLR operator until every column reaches a
-relative residual of 1e-10.  The preconditioner of the timed path is the FP64
-recSM factorization: the plain recSM solution alone stops near 1e-7 on this
-system (condition number ~1e7).  The mixed-precision factorization (FP32
-node products) is run once beside it and reported with its outcome.
+with fused kernel assembly; U/V panels sized by the learned rank predictor),
+recSM factorization, solve, and flexible-GMRES refinement against the FP64
+HODLR operator until every column reaches a relative residual of 1e-10
+(evaluated in double-double arithmetic).  The preconditioner is the FP64
+recSM factorization: the plain recSM solution alone stops near 1e-6 on this
+system (condition number ~1e7).  Beside it: the same solve without the
+predictor (the plain FP64 path), the mixed-precision factorization (FP32
+node products) run once with its outcome, and the predictor deployed on a
+Matern-3/2 system it was not trained on.
 
 Prints one JSON line: the contract fields, the roofline of the dominant
 kernels (the ACA row and column residual passes, HBM bound, timed with HIP
@@ -48,6 +51,7 @@ def parse():
     p.add_argument("--no-cpu-baseline", action="store_true")
     p.add_argument("--no-mixed", action="store_true")
     p.add_argument("--no-roofline", action="store_true")
+    p.add_argument("--no-learned-deploy", action="store_true")
     p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
     p.add_argument("--copy-kernel", action="store_true",
                    help="only run a device-to-device copy of 1 GiB five times (calibrates the "
@@ -66,8 +70,10 @@ def copy_kernel():
                       "launches": 5}), flush=True)
 
 
+from dense_hodlr_amd import inputs as I  # noqa: E402
+
+
 def workload(n, nrhs, seed=12345):
-    from dense_hodlr_amd import inputs as I
     return I.points_2d_kd(n, seed), I.rhs_normal(n, nrhs, seed + 1)
 
 
@@ -139,8 +145,13 @@ def main():
     dX = torch.empty_like(dB)
     torch.cuda.synchronize()
 
-    def step(precision="fp64_refined", max_steps=30):
+    from dense_hodlr_amd import RankModel
+    model = RankModel.load()
+
+    def step(precision="fp64_refined", max_steps=30, learned=True):
         g = make_handle(pts, a.target, precision, max_steps)
+        if learned:
+            g.set_rank_predictor(model, 0.25)
         wall = solve_device(g, dB.data_ptr(), dX.data_ptr(), n, nrhs)
         return g, wall
 
@@ -168,13 +179,25 @@ def main():
     res64 = float(np.max(np.linalg.norm(B - g.matvec(X), axis=0) / bn))
     g = None
 
+    # the plain FP64 path: the same solve without the rank predictor (panels
+    # grown adaptively from 32 columns), same number of steps
+    pwalls, pst = [], None
+    for _ in range(a.steps):
+        g, wall = step(learned=False)
+        pst = g.stats()
+        pwalls.append(wall)
+        g = None
+    plain = float(np.median(pwalls))
+
     out = {
         "metric": METRIC, "value": value, "unit": "s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms, "higher_is_better": False, "scaling": "weak",
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"gauss2d_kd N={n} gaussian l=0.05 shift=1e-4 leaf=256 "
                                f"aca_tol=1e-10 nrhs={nrhs} target={a.target:g}",
-                   "path": "plain FP64: ACA + recSM factor + solve + FGMRES refinement"},
+                   "path": "learned rank sizing + FP64 recSM + FGMRES refinement with "
+                           "double-double residuals (the FP32 factorization does not "
+                           "converge at this condition number: see mixed)"},
         "phases_s": {"compress": st["compress_time"], "factor": st["factor_time"],
                      "solve_and_refine": st["solve_time"]},
         "refine_steps": st["refine_iterations"],
@@ -183,6 +206,12 @@ def main():
         "residual_fp64_evaluation": res64,
         "ranks_max_per_level": st["max_rank"],
         "aca_iterations": st["aca_iterations"],
+        "predictor": {"blocks": st["pred_blocks"], "misses": st["pred_misses"],
+                      "mean_abs_log_rank_error": st["pred_log_err"]},
+        "plain_fp64": {"value": plain, "compress_s": pst["compress_time"],
+                       "factor_s": pst["factor_time"], "solve_and_refine_s": pst["solve_time"],
+                       "residual": pst["last_residual"]},
+        "speedup_vs_plain": plain / value,
     }
 
     if not a.no_mixed:
@@ -199,6 +228,30 @@ def main():
         except HodlrError as e:
             out["mixed"] = {"reached_target": False, "error": str(e)}
         del g
+
+    if not a.no_learned_deploy:
+        # the predictor on a system it was not trained on: Matern-3/2, l = 0.1,
+        # N = 262144, tolerance 1e-10; compression only, with and without it
+        dpts = I.points_2d_kd(n, 4242)
+        times = {}
+        for learned in (False, True, False, True):
+            from dense_hodlr_amd import HODLR_Matrix
+            g = HODLR_Matrix.from_kernel(dpts, "matern32", 0.1, 1e-3, sizeThreshold=256)
+            g.set_LRTolerance(1e-10)
+            if learned:
+                g.set_rank_predictor(model, 0.25)
+            t0 = time.perf_counter()
+            g.storeLRinTree()
+            times.setdefault(learned, []).append(time.perf_counter() - t0)
+            sd = g.stats()
+            g = None
+        out["learned_deploy"] = {
+            "workload": f"matern32 2D kd N={n} l=0.1 leaf=256 tol=1e-10 (compression)",
+            "plain_aca_s": min(times[False]), "predictor_sized_s": min(times[True]),
+            "speedup": min(times[False]) / min(times[True]),
+            "blocks": sd["pred_blocks"], "misses": sd["pred_misses"],
+            "miss_rate": sd["pred_misses"] / max(1, sd["pred_blocks"]),
+            "mean_abs_log_rank_error": sd["pred_log_err"], "max_rank": max(sd["max_rank"])}
 
     if not a.no_roofline:
         # one extra compression with HIP events around every ACA row/column

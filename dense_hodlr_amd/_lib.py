@@ -35,6 +35,10 @@ EXPORTS = [
     "hodlr_matvec", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
     "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
     "hodlr_create_batch", "hodlr_batch_count", "hodlr_batch_offsets", "hodlr_logabsdet_batch",
+    "hodlr_rank_model_create", "hodlr_rank_model_destroy", "hodlr_rank_model_num_params",
+    "hodlr_rank_model_get_params", "hodlr_rank_model_set_params", "hodlr_rank_model_forward",
+    "hodlr_rank_model_predict_bf16", "hodlr_rank_model_gradient", "hodlr_rank_model_train",
+    "hodlr_block_features", "hodlr_set_rank_predictor",
 ]
 
 
@@ -68,6 +72,9 @@ class Stats(C.Structure):
         ("aca_col_time", C.c_double),
         ("aca_launches", C.c_int),
         ("aca_bytes", C.c_double),
+        ("pred_blocks", C.c_int),
+        ("pred_misses", C.c_int),
+        ("pred_log_err", C.c_double),
     ]
 
 
@@ -92,6 +99,7 @@ def lib():
     L = C.CDLL(LIB_PATH)
     dp = C.POINTER(C.c_double)
     ip = C.POINTER(C.c_int)
+    fp = C.POINTER(C.c_float)
     h = C.c_void_p
     hp = C.POINTER(C.c_void_p)
     sig = {
@@ -125,6 +133,18 @@ def lib():
         "hodlr_batch_count": (C.c_int, [h]),
         "hodlr_batch_offsets": (C.c_int, [h, ip]),
         "hodlr_logabsdet_batch": (C.c_int, [h, dp]),
+        "hodlr_rank_model_create": (C.c_int, [hp, C.c_ulonglong]),
+        "hodlr_rank_model_destroy": (C.c_int, [h]),
+        "hodlr_rank_model_num_params": (C.c_int, []),
+        "hodlr_rank_model_get_params": (C.c_int, [h, fp]),
+        "hodlr_rank_model_set_params": (C.c_int, [h, fp]),
+        "hodlr_rank_model_forward": (C.c_int, [h, fp, C.c_int, fp]),
+        "hodlr_rank_model_predict_bf16": (C.c_int, [h, fp, C.c_int, fp]),
+        "hodlr_rank_model_gradient": (C.c_int, [h, fp, fp, C.c_int, fp, fp]),
+        "hodlr_rank_model_train": (C.c_int, [h, fp, fp, C.c_int, C.c_int, C.c_int, C.c_float,
+                                             C.c_ulonglong, fp]),
+        "hodlr_block_features": (C.c_int, [h, C.c_double, fp]),
+        "hodlr_set_rank_predictor": (C.c_int, [h, h, C.c_double]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

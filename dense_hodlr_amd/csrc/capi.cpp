@@ -11,10 +11,15 @@
 
 #include "../../include/dense_hodlr.h"
 #include "engine.h"
+#include "model.h"
 
 struct hodlr_handle_s {
   std::unique_ptr<hodlr::Engine> eng;
   std::string err;
+};
+
+struct hodlr_rank_model_s {
+  std::unique_ptr<hodlr::RankModel> m;
 };
 
 namespace {
@@ -47,6 +52,23 @@ int guard(hodlr_handle h, F&& f) {
     return fail(h, HODLR_ERR_NOMEM, "host allocation failed");
   } catch (const std::exception& e) {
     return fail(h, HODLR_ERR_STATE, e.what());
+  }
+}
+
+template <class F>
+int guard_model(hodlr_rank_model m, F&& f) {
+  if (!m) return fail(nullptr, HODLR_ERR_ARG, "null model");
+  try {
+    f();
+    return HODLR_OK;
+  } catch (const std::invalid_argument& e) {
+    return fail(nullptr, HODLR_ERR_ARG, e.what());
+  } catch (const hodlr::HipError& e) {
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(nullptr, HODLR_ERR_STATE, e.what());
   }
 }
 
@@ -297,6 +319,91 @@ int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const dou
   });
 }
 
+int hodlr_rank_model_create(hodlr_rank_model* m, unsigned long long seed) {
+  if (!m) return fail(nullptr, HODLR_ERR_ARG, "null model pointer");
+  *m = nullptr;
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  auto* mm = new (std::nothrow) hodlr_rank_model_s();
+  if (!mm) return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
+  try {
+    mm->m.reset(new hodlr::RankModel(seed));
+  } catch (const std::exception& e) {
+    delete mm;
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  }
+  *m = mm;
+  return HODLR_OK;
+}
+
+int hodlr_rank_model_destroy(hodlr_rank_model m) {
+  if (!m) return fail(nullptr, HODLR_ERR_ARG, "null model");
+  delete m;
+  return HODLR_OK;
+}
+
+int hodlr_rank_model_num_params(void) { return hodlr::MLP_NPARAM; }
+
+int hodlr_rank_model_get_params(hodlr_rank_model m, float* out) {
+  return guard_model(m, [&] {
+    if (!out) throw std::invalid_argument("null output");
+    m->m->get_params(out);
+  });
+}
+
+int hodlr_rank_model_set_params(hodlr_rank_model m, const float* in) {
+  return guard_model(m, [&] {
+    if (!in) throw std::invalid_argument("null input");
+    m->m->set_params(in);
+  });
+}
+
+int hodlr_rank_model_forward(hodlr_rank_model m, const float* X, int n, float* y) {
+  return guard_model(m, [&] {
+    if (n < 0 || (n > 0 && (!X || !y))) throw std::invalid_argument("bad arguments");
+    m->m->forward(X, n, y);
+  });
+}
+
+int hodlr_rank_model_predict_bf16(hodlr_rank_model m, const float* X, int n, float* y) {
+  return guard_model(m, [&] {
+    if (n < 0 || (n > 0 && (!X || !y))) throw std::invalid_argument("bad arguments");
+    m->m->predict_bf16(X, n, y);
+  });
+}
+
+int hodlr_rank_model_gradient(hodlr_rank_model m, const float* X, const float* t, int n,
+                              float* grad, float* loss) {
+  return guard_model(m, [&] {
+    if (n <= 0 || !X || !t || !grad) throw std::invalid_argument("bad arguments");
+    const float l = m->m->gradient(X, t, n, grad);
+    if (loss) *loss = l;
+  });
+}
+
+int hodlr_rank_model_train(hodlr_rank_model m, const float* X, const float* t, int n, int epochs,
+                           int batch, float lr, unsigned long long seed, float* epoch_loss) {
+  return guard_model(m, [&] {
+    if (!X || !t) throw std::invalid_argument("null data");
+    m->m->train(X, t, n, epochs, batch, lr, seed, epoch_loss);
+  });
+}
+
+int hodlr_block_features(hodlr_handle h, double tol, float* out) {
+  return guard(h, [&] {
+    if (!out || !(tol > 0.0)) throw std::invalid_argument("bad arguments");
+    h->eng->block_features(tol, out);
+  });
+}
+
+int hodlr_set_rank_predictor(hodlr_handle h, hodlr_rank_model m, double margin) {
+  return guard(h, [&] {
+    if (m)
+      h->eng->set_rank_predictor(m->m->host_params().data(), margin);
+    else
+      h->eng->set_rank_predictor(nullptr, margin);
+  });
+}
+
 int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
   return guard(h, [&] {
     if (!out) throw std::invalid_argument("null output");
@@ -307,6 +414,9 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->num_leaves = (int)e.leaves().size();
     const auto& s = e.stats();
     out->aca_iterations = s.aca_iterations;
+    out->pred_blocks = s.pred_blocks;
+    out->pred_misses = s.pred_misses;
+    out->pred_log_err = s.pred_log_err;
     out->compress_time = s.compress_time;
     out->factor_time = s.factor_time;
     out->solve_time = s.solve_time;

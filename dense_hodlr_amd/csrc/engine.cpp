@@ -2,6 +2,7 @@
  * Host orchestration of the HODLR engine (see engine.h for the layout).
  */
 #include "engine.h"
+#include "mlp.h"
 #include "svd.h"
 
 #include <algorithm>
@@ -515,10 +516,17 @@ void Engine::aca_compress() {
     L.P[l] = P[l].p;
     L.cap[l] = c;
   };
+  // initial panel capacity per level: the rank predictor's capacities when
+  // one is attached (all blocks of kernel instances), else a fixed start
+  std::vector<int> pcap;
+  if (!pred_params_.empty()) pcap = predicted_capacities();
+  std::vector<int> lvl_pcap(nl, 0);
+  for (int b = 0; b < (int)pcap.size(); b++)
+    lvl_pcap[blk[b].level] = std::max(lvl_pcap[blk[b].level], pcap[b]);
   for (int l = 0; l < nl; l++) {
     if (lvl_count[l] == 0) continue;
     int c = init_cap_ > 0 ? init_cap_ : 32;
-    if (const char* e = std::getenv("HODLR_ACA_INIT_CAP")) c = std::max(1, std::atoi(e));
+    if (!pcap.empty()) c = lvl_pcap[l] + 1;  // + the speculative column
     c = std::max(1, std::min(c, upper_max[l]));
     alloc_level(l, c);
   }
@@ -674,6 +682,16 @@ void Engine::aca_compress() {
   off_[0] = 0;
   for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
   ktot_ = off_[nlev_];
+  if (!pcap.empty()) {  // how the predictor did: misses need the adaptive growth
+    stats_.pred_blocks = nb;
+    stats_.pred_misses = 0;
+    double err = 0.0;
+    for (int b = 0; b < nb; b++) {
+      if (ranks[b] > pcap[b]) stats_.pred_misses++;
+      err += std::fabs(std::log(std::max(pred_rank_[b], 1.0)) - std::log((double)std::max(ranks[b], 1)));
+    }
+    stats_.pred_log_err = err / nb;
+  }
   // algorithmic bytes of the residual passes: 8 (m+n) k(k+1)/2 per block
   double bytes = 0;
   for (int b = 0; b < nb; b++) {
@@ -858,6 +876,81 @@ void Engine::svd_compress() {
   for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
   ktot_ = off_[nlev_];
   stats_.aca_bytes = 0;
+}
+
+/* Features of every off-diagonal block (2 per node, node order: top, then
+ * bottom) for the rank predictor; empty when an instance has no points. */
+void Engine::block_features_device(double tol, DBuf<float>& out) {
+  const int nn = (int)nodes_.size();
+  std::vector<FeatDesc> fd;
+  for (int id = 0; id < nn; id++) {
+    const TreeNodeH& nd = nodes_[id];
+    const EntrySource& es = h_srcs_[nd.inst];
+    if (es.kind != 0) throw std::invalid_argument("block features need a kernel (point) instance");
+    for (int h = 0; h < 2; h++) {
+      FeatDesc f{};
+      f.x = es.kp.x;
+      f.y = es.kp.y;
+      f.z = es.kp.z;
+      const int top0 = nd.min_i, bot0 = nd.split + 1;
+      const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
+      f.r0 = h == 0 ? top0 : bot0;
+      f.nr = h == 0 ? st : sb;
+      f.c0 = h == 0 ? bot0 : top0;
+      f.nc = h == 0 ? sb : st;
+      f.dim = es.kp.dim;
+      f.level = nd.level;
+      f.length_scale = (float)(1.0 / es.kp.inv_l);
+      f.log10_tol = (float)std::log10(tol);
+      f.family = (float)es.kp.family;
+      fd.push_back(f);
+    }
+  }
+  out.alloc(std::max<size_t>(1, fd.size() * MLP_IN));
+  launch_block_features(arena_.push(fd, stream_), (int)fd.size(), out.p, stream_);
+}
+
+void Engine::block_features(double tol, float* host_out) {
+  DBuf<float> f;
+  block_features_device(tol, f);
+  HODLR_HIP_CHECK(hipMemcpyAsync(host_out, f.p, nodes_.size() * 2 * MLP_IN * sizeof(float),
+                                 hipMemcpyDeviceToHost, stream_));
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  arena_.reset();
+}
+
+void Engine::set_rank_predictor(const float* params, double margin) {
+  if (params) {
+    if (!(margin >= 0.0)) throw std::invalid_argument("margin must be >= 0");
+    pred_params_.assign(params, params + MLP_NPARAM);
+  } else {
+    pred_params_.clear();
+  }
+  pred_margin_ = margin;
+  reset_attributes();
+}
+
+/* Rank capacity of every block from the predictor (BF16 inference):
+ * ceil(exp(y) (1 + margin)), y the predicted log-rank. */
+std::vector<int> Engine::predicted_capacities() {
+  const int nb = 2 * (int)nodes_.size();
+  DBuf<float> feat, P, y;
+  block_features_device(tol_, feat);
+  P.alloc(MLP_NPARAM);
+  y.alloc(std::max(nb, 1));
+  HODLR_HIP_CHECK(hipMemcpyAsync(P.p, pred_params_.data(), MLP_NPARAM * sizeof(float), hipMemcpyHostToDevice, stream_));
+  launch_mlp_infer_bf16(P.p, feat.p, nb, y.p, stream_);
+  std::vector<float> hy(nb);
+  HODLR_HIP_CHECK(hipMemcpyAsync(hy.data(), y.p, nb * sizeof(float), hipMemcpyDeviceToHost, stream_));
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+  std::vector<int> cap(nb);
+  pred_rank_.assign(nb, 0.0);
+  for (int b = 0; b < nb; b++) {
+    const double r = std::exp(std::min(20.0, (double)hy[b]));
+    pred_rank_[b] = r;
+    cap[b] = std::max(1, (int)std::ceil(r * (1.0 + pred_margin_)));
+  }
+  return cap;
 }
 
 void Engine::assemble_leaves() {

@@ -179,8 +179,13 @@ class Engine {
     double aca_row_time = 0, aca_col_time = 0;  // with profiling on
     int aca_launches = 0;
     double aca_bytes = 0;
+    int pred_blocks = 0, pred_misses = 0;  // with a rank predictor attached
+    double pred_log_err = 0;               // mean |log predicted - log ACA rank|
   };
   void set_profiling(bool on) { profiling_ = on; }
+  // rank predictor (MLP parameters, host copy); null detaches it
+  void set_rank_predictor(const float* params, double margin);
+  void block_features(double tol, float* host_out);
   const Stats& stats() const { return stats_; }
   long long device_bytes() const;
   hipStream_t stream() const { return stream_; }
@@ -191,6 +196,8 @@ class Engine {
   void reset_attributes();
   void aca_compress();
   void svd_compress();
+  void block_features_device(double tol, DBuf<float>& out);
+  std::vector<int> predicted_capacities();
   void assemble_leaves();
   void factor_leaves();
   void run_gemm(std::vector<GemmDesc>& descs);
@@ -227,6 +234,9 @@ class Engine {
   int refine_max_ = 30;
   int init_cap_ = 0;
   bool profiling_ = false;
+  std::vector<float> pred_params_;
+  double pred_margin_ = 0.25;
+  std::vector<double> pred_rank_;
   bool trace_ = false;
   std::vector<hipEvent_t> prof_ev_;
 

@@ -103,6 +103,11 @@ class HODLR_Matrix:
     def set_profiling(self, on: bool = True):
         check(lib().hodlr_set_profiling(self._h, int(bool(on))), self._h)
 
+    def set_rank_predictor(self, model=None, margin: float = 0.25):
+        """Size compression from a RankModel's predictions (None detaches)."""
+        check(lib().hodlr_set_rank_predictor(self._h, model._h if model is not None else None,
+                                             float(margin)), self._h)
+
     def set_refinement(self, target: float = 1e-10, max_iters: int = 30):
         check(lib().hodlr_set_refinement(self._h, float(target), int(max_iters)), self._h)
 
@@ -174,7 +179,9 @@ class HODLR_Matrix:
                     last_residual=s.last_residual, max_rank=list(s.max_rank[:L]),
                     mean_rank=list(s.mean_rank[:L]), device_bytes=s.device_bytes,
                     aca_row_time=s.aca_row_time, aca_col_time=s.aca_col_time,
-                    aca_launches=s.aca_launches, aca_bytes=s.aca_bytes)
+                    aca_launches=s.aca_launches, aca_bytes=s.aca_bytes,
+                    pred_blocks=s.pred_blocks, pred_misses=s.pred_misses,
+                    pred_log_err=s.pred_log_err)
 
     def get_LR_ComputationTime(self) -> float:
         return self.stats()["compress_time"]

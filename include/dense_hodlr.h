@@ -103,6 +103,12 @@ typedef struct hodlr_stats {
   double aca_col_time;
   int aca_launches;
   double aca_bytes;
+  /* with a rank predictor attached (hodlr_set_rank_predictor): blocks sized
+   * by it, blocks whose ACA rank exceeded the predicted capacity (the
+   * adaptive growth took over), mean |log predicted - log ACA rank| */
+  int pred_blocks;
+  int pred_misses;
+  double pred_log_err;
 } hodlr_stats;
 
 /* ---- construction ------------------------------------------------------ */
@@ -198,6 +204,49 @@ int hodlr_batch_offsets(hodlr_handle h, int* row0);
 /* log|det| of each instance (count entries), as hodlr_logabsdet computes it
  * for one matrix. */
 int hodlr_logabsdet_batch(hodlr_handle h, double* out);
+
+/* ---- learned rank predictor (no reference counterpart) ------------------ */
+
+/* A small MLP, 8 -> 64 -> 64 -> 1 with GELU, that predicts the log of a
+ * block's ACA rank from its geometry (the 8 features of
+ * hodlr_block_features).  FP32 parameters on the GPU; training (Adam on the
+ * mean squared error) and inference run in the library's own HIP kernels,
+ * inference in BF16.  Errors of the model calls are reported by
+ * hodlr_last_error(NULL). */
+typedef struct hodlr_rank_model_s* hodlr_rank_model;
+
+/* Glorot-uniform weights, zero biases, from a seeded generator. */
+int hodlr_rank_model_create(hodlr_rank_model* m, unsigned long long seed);
+int hodlr_rank_model_destroy(hodlr_rank_model m);
+/* Parameter count (4801): W1[64][8], b1[64], W2[64][64], b2[64], W3[64], b3. */
+int hodlr_rank_model_num_params(void);
+int hodlr_rank_model_get_params(hodlr_rank_model m, float* out);
+/* Sets the parameters and clears the optimizer state. */
+int hodlr_rank_model_set_params(hodlr_rank_model m, const float* in);
+/* y[i] = f(X[i, :]) for n samples of 8 features (row-major), FP32. */
+int hodlr_rank_model_forward(hodlr_rank_model m, const float* X, int n, float* y);
+/* The same in BF16 arithmetic (parameters and activations rounded to
+ * bfloat16, FP32 accumulation): the inference used to size compression. */
+int hodlr_rank_model_predict_bf16(hodlr_rank_model m, const float* X, int n, float* y);
+/* Gradient of mean((f(X) - t)^2) over the n samples (num_params floats) and
+ * the loss. */
+int hodlr_rank_model_gradient(hodlr_rank_model m, const float* X, const float* t, int n,
+                              float* grad, float* loss);
+/* Adam (lr, beta 0.9 / 0.999) on mini-batches of a seeded permutation per
+ * epoch; epoch_loss (epochs entries, may be NULL) gets the full-data loss. */
+int hodlr_rank_model_train(hodlr_rank_model m, const float* X, const float* t, int n, int epochs,
+                           int batch, float lr, unsigned long long seed, float* epoch_loss);
+/* Features of every off-diagonal block of a kernel handle, 2 per node in
+ * hodlr_get_nodes order (top block, then bottom), 8 floats each:
+ * log(m + n), centroid distance / (radius_rows + radius_cols),
+ * radius_rows / l, radius_cols / l, log(l), log10(tol), level, family. */
+int hodlr_block_features(hodlr_handle h, double tol, float* out);
+/* Attach a copy of the model's current parameters (NULL detaches): the next
+ * compression sizes every level's U/V panels from the predicted ranks times
+ * (1 + margin) before ACA starts; ACA still decides every rank, and blocks
+ * whose rank exceeds the prediction grow the panels as without a predictor.
+ * Drops cached compression, like the other setters. */
+int hodlr_set_rank_predictor(hodlr_handle h, hodlr_rank_model m, double margin);
 
 /* ---- inspection -------------------------------------------------------- */
 

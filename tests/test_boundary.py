@@ -15,7 +15,7 @@ ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def _declared():
     txt = open(os.path.join(ROOT, "include", "dense_hodlr.h")).read()
-    return sorted(set(re.findall(r"\b(hodlr_[a-z_]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b(hodlr_[a-z0-9_]+)\s*\(", txt)))
 
 
 def test_header_and_binding_agree():
