@@ -239,6 +239,10 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
   return guard(h, [&] { h->eng->set_refinement(target, max_iters); });
 }
 
+int hodlr_set_graphs(hodlr_handle h, int on) {
+  return guard(h, [&] { h->eng->set_graphs(on != 0); });
+}
+
 int hodlr_set_profiling(hodlr_handle h, int on) {
   return guard(h, [&] { h->eng->set_profiling(on != 0); });
 }
@@ -414,6 +418,8 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->num_leaves = (int)e.leaves().size();
     const auto& s = e.stats();
     out->aca_iterations = s.aca_iterations;
+    out->graph_captures = s.graph_captures;
+    out->graph_launches = s.graph_launches;
     out->pred_blocks = s.pred_blocks;
     out->pred_misses = s.pred_misses;
     out->pred_log_err = s.pred_log_err;

@@ -21,6 +21,12 @@ T* DescArena::push(const std::vector<T>& v, hipStream_t s) {
   const size_t bytes = v.size() * sizeof(T);
   const size_t need = (bytes + 255) & ~size_t(255);
   if (need == 0) return nullptr;
+  if (replay_) {
+    if (next_ >= keep_.size() || keep_[next_].size() != bytes ||
+        std::memcmp(keep_[next_].data(), v.data(), bytes) != 0)
+      throw std::runtime_error("captured solve does not match its recording");
+    return reinterpret_cast<T*>(dev_[next_++]);
+  }
   if (chunks_.empty() || chunks_.back().used + need > chunks_.back().buf.n) {
     Chunk c;
     c.buf.alloc(std::max<size_t>(need, 8u << 20));
@@ -30,6 +36,7 @@ T* DescArena::push(const std::vector<T>& v, hipStream_t s) {
   keep_.emplace_back(reinterpret_cast<const char*>(v.data()),
                      reinterpret_cast<const char*>(v.data()) + bytes);
   char* dst = c.buf.p + c.used;
+  dev_.push_back(dst);
   HODLR_HIP_CHECK(hipMemcpyAsync(dst, keep_.back().data(), bytes, hipMemcpyHostToDevice, s));
   c.used += need;
   return reinterpret_cast<T*>(dst);
@@ -39,6 +46,8 @@ void DescArena::reset() {
   for (auto& c : chunks_) c.used = 0;
   while (chunks_.size() > 1) chunks_.pop_front();
   keep_.clear();
+  dev_.clear();
+  replay_ = false;
 }
 
 size_t DescArena::bytes() const {
@@ -184,6 +193,8 @@ void Engine::upload_sources() {
 /* settings (reset_attributes semantics, HODLR_Matrix.cpp:1232-1286)    */
 /* ------------------------------------------------------------------ */
 void Engine::reset_attributes() {
+  buf_gen_++;
+  graph_.release();
   compressed_ = false;
   leaves_factored_ = false;
   for (auto& f : fac_) {
@@ -269,6 +280,7 @@ void Engine::ensure_ws(size_t doubles) {
   if (ws_.n < doubles) {
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));  // queued kernels may use the old buffer
     ws_.alloc(doubles + doubles / 4);
+    buf_gen_++;
   }
 }
 
@@ -276,6 +288,7 @@ void Engine::grow(DBuf<double>& b, size_t doubles) {
   if (b.n < doubles) {
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
     b.alloc(doubles);
+    buf_gen_++;
   }
 }
 
@@ -1014,6 +1027,7 @@ void Engine::factorize_fp64(int which) {
   compress();
   Factorization& F = fac_[which];
   if (F.done) return;
+  buf_gen_++;  // captured solves refer to the factorization's buffers
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
   factor_leaves();
   const size_t N = (size_t)ntot_;
@@ -1152,7 +1166,10 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
   grow(xw_, N * nrhs);
   HODLR_HIP_CHECK(hipMemcpy2DAsync(xw_.p, N * sizeof(double), dB, (size_t)ldb * sizeof(double),
                                    N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
-  solve_fp64_inplace(which, xw_.p, nrhs);
+  if (use_graphs_)
+    solve_fp64_graph(which, xw_.p, nrhs);
+  else
+    solve_fp64_inplace(which, xw_.p, nrhs);
   HODLR_HIP_CHECK(hipMemcpy2DAsync(dX, (size_t)ldx * sizeof(double), xw_.p, N * sizeof(double),
                                    N * sizeof(double), nrhs, hipMemcpyDeviceToDevice, stream_));
   HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
@@ -1161,6 +1178,67 @@ void Engine::solve_device(int which, const double* dB, int ldb, int nrhs, double
   HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
   stats_.solve_time = ms * 1e-3;
   arena_.reset();
+}
+
+void Engine::set_graphs(bool on) {
+  use_graphs_ = on;
+  if (!on) graph_.release();
+}
+
+/* The FP64 solve through a captured hipGraph.  On first use for a
+ * (factorization, nrhs) and whenever a work buffer or the factorization has
+ * been reallocated since, the solve runs once eagerly with its descriptor
+ * arrays uploaded into the graph's own arena, and is then captured with
+ * stream capture while that arena replays the same device pointers (no copy
+ * is captured, only kernel launches).  Later calls copy X into the graph's
+ * buffer, launch the graph and copy the result back. */
+void Engine::solve_fp64_graph(int which, double* X, int nrhs) {
+  const size_t N = (size_t)ntot_;
+  grow(gx_, N * nrhs);
+  const size_t bytes = N * nrhs * sizeof(double);
+  HODLR_HIP_CHECK(hipMemcpyAsync(gx_.p, X, bytes, hipMemcpyDeviceToDevice, stream_));
+  const bool stale = !graph_.valid || graph_.which != which || graph_.nrhs != nrhs ||
+                     graph_.gen != buf_gen_;
+  if (!stale) {
+    HODLR_HIP_CHECK(hipGraphLaunch(graph_.exec, stream_));
+    stats_.graph_launches++;
+  } else {
+    graph_.release();
+    graph_.arena.reset();
+    std::swap(arena_, graph_.arena);
+    try {
+      solve_fp64_inplace(which, gx_.p, nrhs);  // eager run: the result of this call
+      HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+      const unsigned long long gen = buf_gen_;
+      arena_.begin_replay();
+      HODLR_HIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeGlobal));
+      hipGraph_t g = nullptr;
+      try {
+        solve_fp64_inplace(which, gx_.p, nrhs);
+      } catch (...) {
+        (void)hipStreamEndCapture(stream_, &g);
+        if (g) (void)hipGraphDestroy(g);
+        throw;
+      }
+      HODLR_HIP_CHECK(hipStreamEndCapture(stream_, &g));
+      graph_.graph = g;
+      HODLR_HIP_CHECK(hipGraphInstantiate(&graph_.exec, g, nullptr, nullptr, 0));
+      if (buf_gen_ != gen) throw std::runtime_error("work buffers changed during capture");
+      graph_.valid = true;
+      graph_.which = which;
+      graph_.nrhs = nrhs;
+      graph_.gen = gen;
+      stats_.graph_captures++;
+    } catch (...) {
+      arena_.end_replay();
+      std::swap(arena_, graph_.arena);
+      graph_.release();
+      throw;
+    }
+    arena_.end_replay();
+    std::swap(arena_, graph_.arena);
+  }
+  HODLR_HIP_CHECK(hipMemcpyAsync(X, gx_.p, bytes, hipMemcpyDeviceToDevice, stream_));
 }
 
 /* X <- A_H^-1 X with the FP64 recSM (recLU) factorization, X is ntot x nrhs
@@ -1616,7 +1694,10 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     } else {
       if (in != out)
         HODLR_HIP_CHECK(hipMemcpyAsync(out, in, N * nrhs * sizeof(double), hipMemcpyDeviceToDevice, stream_));
-      solve_fp64_inplace(which, out, nrhs);
+      if (use_graphs_)
+        solve_fp64_graph(which, out, nrhs);
+      else
+        solve_fp64_inplace(which, out, nrhs);
     }
   };
   std::vector<double> bn, rs, rn(nrhs), hn, coef((size_t)(m + 1) * nrhs);

@@ -86,6 +86,14 @@ class DescArena {
   T* push(const std::vector<T>& v, hipStream_t s);
   void reset();
   size_t bytes() const;
+  /* Replay mode (for stream capture): push() copies nothing and returns, in
+   * order, the device pointers of the pushes made since the last reset,
+   * after checking that the contents are the same. */
+  void begin_replay() {
+    replay_ = true;
+    next_ = 0;
+  }
+  void end_replay() { replay_ = false; }
 
  private:
   struct Chunk {
@@ -94,6 +102,28 @@ class DescArena {
   };
   std::deque<Chunk> chunks_;
   std::deque<std::vector<char>> keep_;
+  std::vector<char*> dev_;  // device copy of keep_[i]
+  bool replay_ = false;
+  size_t next_ = 0;
+};
+
+/* A captured hipGraph of the FP64 recSM/recLU solve for one (factorization,
+ * number of right-hand sides), working in place on the engine's buffer gx_. */
+struct SolveGraph {
+  bool valid = false;
+  int which = -1, nrhs = 0;
+  unsigned long long gen = 0;  // work-buffer generation it was captured with
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  DescArena arena;  // device descriptors the captured kernels read
+  void release() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    valid = false;
+  }
+  ~SolveGraph() { release(); }
 };
 
 struct InstanceSpec {
@@ -179,10 +209,14 @@ class Engine {
     double aca_row_time = 0, aca_col_time = 0;  // with profiling on
     int aca_launches = 0;
     double aca_bytes = 0;
+    int graph_captures = 0, graph_launches = 0;
     int pred_blocks = 0, pred_misses = 0;  // with a rank predictor attached
     double pred_log_err = 0;               // mean |log predicted - log ACA rank|
   };
   void set_profiling(bool on) { profiling_ = on; }
+  // replay FP64 solves (and FP64-preconditioned refinement steps) from
+  // captured hipGraphs of the level sequence
+  void set_graphs(bool on);
   // rank predictor (MLP parameters, host copy); null detaches it
   void set_rank_predictor(const float* params, double margin);
   void block_features(double tol, float* host_out);
@@ -210,6 +244,8 @@ class Engine {
   void logdet_terms(std::vector<double>& hl, std::vector<double>& hp, std::vector<int>& inst);
   void factorize_mixed(int which);
   void solve_fp64_inplace(int which, double* X, int nrhs);
+  /* X (n x nrhs, ld n) <- A^-1 X through the captured graph */
+  void solve_fp64_graph(int which, double* X, int nrhs);
   void solve_mixed_once(int which, float* X, int nrhs);
   void solve_gmres(int which, bool mixed, const double* dB, int ldb, int nrhs, double* dX, int ldx);
   template <class T>
@@ -234,6 +270,10 @@ class Engine {
   int refine_max_ = 30;
   int init_cap_ = 0;
   bool profiling_ = false;
+  bool use_graphs_ = false;
+  unsigned long long buf_gen_ = 0;  // bumped whenever a work buffer is reallocated
+  SolveGraph graph_;
+  DBuf<double> gx_;
   std::vector<float> pred_params_;
   double pred_margin_ = 0.25;
   std::vector<double> pred_rank_;

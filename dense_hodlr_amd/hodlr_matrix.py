@@ -108,6 +108,10 @@ class HODLR_Matrix:
         check(lib().hodlr_set_rank_predictor(self._h, model._h if model is not None else None,
                                              float(margin)), self._h)
 
+    def set_graphs(self, on: bool = True):
+        """Replay FP64 solves from captured hipGraphs."""
+        check(lib().hodlr_set_graphs(self._h, int(bool(on))), self._h)
+
     def set_refinement(self, target: float = 1e-10, max_iters: int = 30):
         check(lib().hodlr_set_refinement(self._h, float(target), int(max_iters)), self._h)
 
@@ -181,7 +185,8 @@ class HODLR_Matrix:
                     aca_row_time=s.aca_row_time, aca_col_time=s.aca_col_time,
                     aca_launches=s.aca_launches, aca_bytes=s.aca_bytes,
                     pred_blocks=s.pred_blocks, pred_misses=s.pred_misses,
-                    pred_log_err=s.pred_log_err)
+                    pred_log_err=s.pred_log_err, graph_captures=s.graph_captures,
+                    graph_launches=s.graph_launches)
 
     def get_LR_ComputationTime(self) -> float:
         return self.stats()["compress_time"]

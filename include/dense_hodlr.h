@@ -109,6 +109,9 @@ typedef struct hodlr_stats {
   int pred_blocks;
   int pred_misses;
   double pred_log_err;
+  /* with hodlr_set_graphs(h, 1): solve graphs captured and graph launches */
+  int graph_captures;
+  int graph_launches;
 } hodlr_stats;
 
 /* ---- construction ------------------------------------------------------ */
@@ -149,6 +152,15 @@ int hodlr_set_precision(hodlr_handle h, int precision);
  * double-double arithmetic, as hodlr_residual does) and the maximum number of
  * flexible GMRES steps (default 30; restart length min(20, max_iters)). */
 int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
+
+/* Replay FP64 solves from captured hipGraphs (off by default).  The level
+ * sequence of a recSM/recLU solve with a given number of right-hand sides
+ * (every kernel launch of every level) is captured once per factorization
+ * and replayed by one graph launch; the FP64-preconditioned refinement
+ * (HODLR_PREC_FP64_REFINED) replays it for every preconditioner
+ * application.  Results are identical to the eager solve.  No reference
+ * counterpart. */
+int hodlr_set_graphs(hodlr_handle h, int on);
 
 /* Record HIP events around the ACA residual launches (for roofline figures);
  * off by default.  No reference counterpart. */

@@ -1,0 +1,97 @@
+"""GPU tests of hipGraph replay of the solve (hodlr_set_graphs): every replay
+is checked bit for bit against the eager solve of an identical handle, with
+eager work on other handles and synchronizations between the replays, and
+recapture is checked after changes of factorization, right-hand-side count
+and settings."""
+import sys
+
+import numpy as np
+import pytest
+
+from dense_hodlr_amd import HODLR_Batch, HODLR_Matrix
+from dense_hodlr_amd import inputs as I
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def _pair(n=6000, leaf=64):
+    pts = I.points_1d_sorted(n, 17)
+    hs = []
+    for graphs in (True, False):
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.08, 1e-2, sizeThreshold=leaf)
+        g.set_LRTolerance(1e-10)
+        g.set_graphs(graphs)
+        hs.append(g)
+    return hs
+
+
+def test_graph_replays_match_eager():
+    _case("graph replays vs eager")
+    g, e = _pair()
+    other = HODLR_Matrix.from_kernel(I.points_1d_sorted(2000, 3), "se", 0.1, 1e-2, sizeThreshold=32)
+    for r in range(4):
+        b = I.rhs_normal(6000, 3, 100 + r)
+        xg = g.recSM_Solve(b)
+        xe = e.recSM_Solve(b)
+        assert np.array_equal(xg, xe), r
+        other.recSM_Solve(I.rhs_normal(2000, 1, r))  # eager work between replays
+    st = g.stats()
+    assert st["graph_captures"] == 1 and st["graph_launches"] == 3, st
+
+
+def test_graph_recaptures_on_changes():
+    _case("graph recapture")
+    g, e = _pair(3000, 32)
+    b1 = I.rhs_normal(3000, 2, 1)
+    b4 = I.rhs_normal(3000, 4, 2)
+    for fn in ("recSM_Solve", "recLU_Solve", "recSM_Solve"):
+        assert np.array_equal(getattr(g, fn)(b1), getattr(e, fn)(b1)), fn
+    assert np.array_equal(g.recSM_Solve(b4), e.recSM_Solve(b4))
+    assert np.array_equal(g.recSM_Solve(b4), e.recSM_Solve(b4))
+    for h in (g, e):
+        h.set_LRTolerance(1e-8)  # drops compression and factorization
+    assert np.array_equal(g.recSM_Solve(b1), e.recSM_Solve(b1))
+    assert np.array_equal(g.recSM_Solve(b1), e.recSM_Solve(b1))
+    # statistics restart with the settings change: one capture, one replay
+    st = g.stats()
+    assert st["graph_captures"] == 1 and st["graph_launches"] == 1, st
+
+
+def test_graph_refinement_matches_eager():
+    _case("graph refinement")
+    n = 16384
+    pts = I.points_2d_kd(n, 5)
+    b = I.rhs_normal(n, 4, 6)
+    xs = []
+    for graphs in (True, False):
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+        g.set_LRTolerance(1e-10)
+        g.set_precision("fp64_refined")
+        g.set_refinement(1e-10, 10)
+        g.set_graphs(graphs)
+        xs.append(g.recSM_Solve(b))
+        xs.append(g.recSM_Solve(b))
+        st = g.stats()
+        assert st["last_residual"] <= 1e-10
+        if graphs:
+            assert st["graph_launches"] >= 1
+    assert np.array_equal(xs[0], xs[2]) and np.array_equal(xs[1], xs[3])
+
+
+def test_graph_batch_matches_eager():
+    _case("graph batch")
+    cfg = I.batch_inputs(32, 1024, 4096, seed=13)
+    B = np.concatenate(cfg["rhs"])
+    res = []
+    for graphs in (True, False):
+        g = HODLR_Batch(cfg["points"], "se", cfg["length_scale"], cfg["diag"], sizeThreshold=64)
+        g.set_LRTolerance(1e-10)
+        g.set_graphs(graphs)
+        res.append([g.recSM_Solve(B * (1 + r)) for r in range(3)])
+    for a, b in zip(*res):
+        assert np.array_equal(a, b)
