@@ -239,6 +239,10 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
   return guard(h, [&] { h->eng->set_refinement(target, max_iters); });
 }
 
+int hodlr_autotune(hodlr_handle h) {
+  return guard(h, [&] { h->eng->autotune(); });
+}
+
 int hodlr_set_graphs(hodlr_handle h, int on) {
   return guard(h, [&] { h->eng->set_graphs(on != 0); });
 }
@@ -418,6 +422,12 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->num_leaves = (int)e.leaves().size();
     const auto& s = e.stats();
     out->aca_iterations = s.aca_iterations;
+    const hodlr::Tuning& tu = e.tuning();
+    out->tune_aca_rows_per_thread = tu.aca_pt;
+    out->tune_gemm_tile_m = tu.gemm_bm;
+    out->tune_gemm_split_target = tu.gemm_target;
+    out->tune_lu_panel = tu.lu_nb;
+    out->tuned = tu.tuned ? 1 : 0;
     out->graph_captures = s.graph_captures;
     out->graph_launches = s.graph_launches;
     out->pred_blocks = s.pred_blocks;

@@ -6,6 +6,8 @@
 #include "svd.h"
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <cmath>
 #include <cstdlib>
 #include <cstdio>
@@ -59,6 +61,11 @@ size_t DescArena::bytes() const {
 /* ------------------------------------------------------------------ */
 /* construction                                                        */
 /* ------------------------------------------------------------------ */
+namespace {
+std::mutex g_tune_mu;
+std::map<std::string, Tuning> g_tune;  // autotuned shapes by problem shape
+}  // namespace
+
 Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   trace_ = std::getenv("HODLR_TRACE") != nullptr;
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
@@ -66,6 +73,62 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
   build_trees();
   upload_sources();
+  // problem shape: instance sizes, leaf sizes and kinds
+  shape_key_ = std::to_string(specs_.size());
+  for (auto& sp : specs_)
+    shape_key_ += ":" + std::to_string(sp.n) + "/" + std::to_string(sp.leaf) + "/" +
+                  std::to_string(sp.kind);
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  auto it = g_tune.find(shape_key_);
+  if (it != g_tune.end()) tune_ = it->second;
+}
+
+void Engine::autotune() {
+  Tuning best = tune_;
+  auto time_compress = [&](const Tuning& t) {
+    tune_ = t;
+    reset_attributes();
+    compress();
+    return stats_.compress_time;
+  };
+  auto time_factor = [&](const Tuning& t) {
+    tune_ = t;
+    for (auto& f : fac_) f.done = false;
+    leaves_factored_ = false;
+    factorize_fp64(0);
+    return stats_.factor_time;
+  };
+  double bt = 1e300;
+  for (int pt : {1, 2, 4}) {
+    Tuning t = best;
+    t.aca_pt = pt;
+    const double v = std::min(time_compress(t), time_compress(t));
+    if (v < bt) {
+      bt = v;
+      best = t;
+    }
+  }
+  time_compress(best);
+  bt = 1e300;
+  Tuning fb = best;
+  const int shapes[][2] = {{0, 2048}, {0, 1024}, {0, 4096}, {64, 2048}, {128, 2048}};
+  for (int nb : {64, 32})
+    for (auto& sh : shapes) {
+      Tuning t = best;
+      t.lu_nb = nb;
+      t.gemm_bm = sh[0];
+      t.gemm_target = sh[1];
+      const double v = std::min(time_factor(t), time_factor(t));
+      if (v < bt) {
+        bt = v;
+        fb = t;
+      }
+    }
+  fb.tuned = true;
+  tune_ = fb;
+  reset_attributes();
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  g_tune[shape_key_] = fb;
 }
 
 Engine::~Engine() {
@@ -297,8 +360,8 @@ void Engine::run_gemm_t(std::vector<GemmDescT<T>>& descs) {
   if (descs.empty()) return;
   std::vector<int> split_ids;
   int ntiles = 0;
-  const int bm = gemm_tile_m(descs);
-  const size_t ws = gemm_plan(descs, split_ids, ntiles, 2048, bm);
+  const int bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(descs);
+  const size_t ws = gemm_plan(descs, split_ids, ntiles, tune_.gemm_target, bm);
   if (ws) ensure_ws((ws * sizeof(T) + 7) / 8);
   const GemmDescT<T>* d = arena_.push(descs, stream_);
   const int* ids = split_ids.empty() ? nullptr : arena_.push(split_ids, stream_);
@@ -337,13 +400,13 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
   const LuDesc* dd = arena_.push(descs, stream_);
-  for (int p0 = 0; p0 < maxn; p0 += LU_NB) {
-    launch_lu_panel(dd, (int)descs.size(), p0, LU_NB, stream_);
+  for (int p0 = 0; p0 < maxn; p0 += tune_.lu_nb) {
+    launch_lu_panel(dd, (int)descs.size(), p0, tune_.lu_nb, stream_);
     std::vector<TrsmDesc> tr;
     std::vector<GemmDesc> g;
     int nwg = 0;
     for (auto& d : descs) {
-      const int pw = std::min(LU_NB, d.n - p0);
+      const int pw = std::min(tune_.lu_nb, d.n - p0);
       const int rest = d.n - p0 - pw;
       if (pw <= 0 || rest <= 0) continue;
       TrsmDesc t{};
@@ -392,17 +455,17 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
     launch_permute(arena_.push(pd, stream_), (int)pd.size(), nwg, stream_);
   }
   for (int dir = 0; dir < 2; dir++) {
-    const int nblk = (maxn + LU_NB - 1) / LU_NB;
+    const int nblk = (maxn + tune_.lu_nb - 1) / tune_.lu_nb;
     for (int bi = 0; bi < nblk; bi++) {
       std::vector<TrsmDesc> tr;
       std::vector<GemmDesc> g;
       int nwg = 0;
       for (auto& d : descs) {
-        const int nb_d = (d.n + LU_NB - 1) / LU_NB;
+        const int nb_d = (d.n + tune_.lu_nb - 1) / tune_.lu_nb;
         const int blk = dir == 0 ? bi : nb_d - 1 - bi;  // backward: from the last block
         if (blk < 0 || blk >= nb_d) continue;
-        const int p0 = blk * LU_NB;
-        const int pw = std::min(LU_NB, d.n - p0);
+        const int p0 = blk * tune_.lu_nb;
+        const int pw = std::min(tune_.lu_nb, d.n - p0);
         TrsmDesc t{};
         t.T = d.LU + p0 + (size_t)p0 * d.ld_lu;
         t.B = d.B + p0;
@@ -461,7 +524,7 @@ void Engine::aca_compress() {
     ktot_ = 0;
     return;
   }
-  const int jpt = 2, ipt = 2;
+  const int jpt = tune_.aca_pt, ipt = tune_.aca_pt;
   const int cpw = 256 * jpt, rpw = 256 * ipt;
   const int nb = 2 * nn;
   std::vector<AcaBlock> blk(nb);

@@ -33,7 +33,16 @@
 
 namespace hodlr {
 
-constexpr int LU_NB = 64;  // panel width of the blocked LU and triangular solves
+constexpr int LU_NB = 64;  // largest panel width of the blocked LU and triangular solves
+
+/* Launch shapes chosen by the autotuner (defaults without it). */
+struct Tuning {
+  int aca_pt = 2;          // panel rows per thread of the ACA row/column kernels (1, 2, 4)
+  int gemm_bm = 0;         // grouped GEMM tile rows (64, 128; 0: chosen per launch)
+  int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
+  int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
+  bool tuned = false;
+};
 
 template <class T>
 struct DBuf {
@@ -214,6 +223,12 @@ class Engine {
     double pred_log_err = 0;               // mean |log predicted - log ACA rank|
   };
   void set_profiling(bool on) { profiling_ = on; }
+  /* Times the candidate launch shapes on this problem (compression for the
+   * ACA rows per thread; FP64 factorization for the GEMM tiles, split-K
+   * target and LU panel width), keeps the fastest and records it for every
+   * later handle of the same shape in this process. */
+  void autotune();
+  const Tuning& tuning() const { return tune_; }
   // replay FP64 solves (and FP64-preconditioned refinement steps) from
   // captured hipGraphs of the level sequence
   void set_graphs(bool on);
@@ -271,6 +286,8 @@ class Engine {
   int init_cap_ = 0;
   bool profiling_ = false;
   bool use_graphs_ = false;
+  Tuning tune_;
+  std::string shape_key_;
   unsigned long long buf_gen_ = 0;  // bumped whenever a work buffer is reallocated
   SolveGraph graph_;
   DBuf<double> gx_;

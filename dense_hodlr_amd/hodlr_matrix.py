@@ -108,6 +108,10 @@ class HODLR_Matrix:
         check(lib().hodlr_set_rank_predictor(self._h, model._h if model is not None else None,
                                              float(margin)), self._h)
 
+    def autotune(self):
+        """Pick launch shapes by timing candidates on this problem."""
+        check(lib().hodlr_autotune(self._h), self._h)
+
     def set_graphs(self, on: bool = True):
         """Replay FP64 solves from captured hipGraphs."""
         check(lib().hodlr_set_graphs(self._h, int(bool(on))), self._h)
@@ -186,7 +190,11 @@ class HODLR_Matrix:
                     aca_launches=s.aca_launches, aca_bytes=s.aca_bytes,
                     pred_blocks=s.pred_blocks, pred_misses=s.pred_misses,
                     pred_log_err=s.pred_log_err, graph_captures=s.graph_captures,
-                    graph_launches=s.graph_launches)
+                    graph_launches=s.graph_launches,
+                    tuning=dict(aca_rows_per_thread=s.tune_aca_rows_per_thread,
+                                gemm_tile_m=s.tune_gemm_tile_m,
+                                gemm_split_target=s.tune_gemm_split_target,
+                                lu_panel=s.tune_lu_panel, tuned=bool(s.tuned)))
 
     def get_LR_ComputationTime(self) -> float:
         return self.stats()["compress_time"]

@@ -112,6 +112,14 @@ typedef struct hodlr_stats {
   /* with hodlr_set_graphs(h, 1): solve graphs captured and graph launches */
   int graph_captures;
   int graph_launches;
+  /* launch shapes in use (hodlr_autotune): ACA panel rows per thread, grouped
+   * GEMM tile rows (0: per launch), split-K tile target, LU panel width, and
+   * whether they come from the autotuner */
+  int tune_aca_rows_per_thread;
+  int tune_gemm_tile_m;
+  int tune_gemm_split_target;
+  int tune_lu_panel;
+  int tuned;
 } hodlr_stats;
 
 /* ---- construction ------------------------------------------------------ */
@@ -152,6 +160,14 @@ int hodlr_set_precision(hodlr_handle h, int precision);
  * double-double arithmetic, as hodlr_residual does) and the maximum number of
  * flexible GMRES steps (default 30; restart length min(20, max_iters)). */
 int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
+
+/* Autotuner: times candidate launch shapes on this handle's own problem
+ * (ACA panel rows per thread 1/2/4 on the compression; grouped GEMM tile
+ * rows, split-K target and LU panel width on the FP64 factorization), keeps
+ * the fastest, and applies them to this handle and to every handle of the
+ * same shape (instance sizes, leaf sizes) created later in the process.
+ * Drops cached compression and factorization.  No reference counterpart. */
+int hodlr_autotune(hodlr_handle h);
 
 /* Replay FP64 solves from captured hipGraphs (off by default).  The level
  * sequence of a recSM/recLU solve with a given number of right-hand sides
