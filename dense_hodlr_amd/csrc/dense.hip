@@ -4,6 +4,8 @@
  */
 #include "dense.h"
 
+#include <algorithm>
+
 namespace hodlr {
 
 namespace {
@@ -423,14 +425,20 @@ int grid_x(size_t elems, int ndesc) {
   return (int)g;
 }
 
+/* grid.y is limited to 65535: descriptors are launched in chunks */
+constexpr int MAX_GRID_Y = 65535;
+
 }  // namespace
 
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s) {
   if (count == 0) return;
-  dim3 g(grid_x((size_t)max_elems, count), count);
-  assemble_kernel<<<g, TPB, 0, s>>>(d, srcs);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    const int c = std::min(MAX_GRID_Y, count - c0);
+    dim3 g(grid_x((size_t)max_elems, count), c);
+    assemble_kernel<<<g, TPB, 0, s>>>(d + c0, srcs);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, hipStream_t s) {
@@ -453,16 +461,20 @@ void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s) {
 
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s) {
   if (count == 0) return;
-  dim3 g(16, count);
-  identity_kernel<<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    dim3 g(16, std::min(MAX_GRID_Y, count - c0));
+    identity_kernel<<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_copy(const CopyDesc* d, int count, hipStream_t s) {
   if (count == 0) return;
-  dim3 g(16, count);
-  copy_kernel<<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    dim3 g(16, std::min(MAX_GRID_Y, count - c0));
+    copy_kernel<<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod, hipStream_t s) {
@@ -474,9 +486,11 @@ void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod
 template <class TZ>
 void launch_capacitance(const CapDescT<TZ>* d, int count, hipStream_t s) {
   if (count == 0) return;
-  dim3 g(16, count);
-  capacitance_kernel<TZ><<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    dim3 g(16, std::min(MAX_GRID_Y, count - c0));
+    capacitance_kernel<TZ><<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 template void launch_capacitance<double>(const CapDesc*, int, hipStream_t);
 template void launch_capacitance<float>(const CapDescF*, int, hipStream_t);
@@ -484,9 +498,11 @@ template void launch_capacitance<float>(const CapDescF*, int, hipStream_t);
 template <class S, class D>
 void launch_convert(const ConvDesc<S, D>* d, int count, hipStream_t s) {
   if (count == 0) return;
-  dim3 g(count > 64 ? 16 : 512, count);
-  convert_kernel<S, D><<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    dim3 g(count > 64 ? 16 : 512, std::min(MAX_GRID_Y, count - c0));
+    convert_kernel<S, D><<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 template void launch_convert<double, float>(const ConvDesc<double, float>*, int, hipStream_t);
 template void launch_convert<float, double>(const ConvDesc<float, double>*, int, hipStream_t);
@@ -494,9 +510,11 @@ template void launch_convert<float, float>(const ConvDesc<float, float>*, int, h
 
 void launch_balance(const BalanceDesc* d, int count, int kmax, hipStream_t s) {
   if (count == 0 || kmax == 0) return;
-  dim3 g(kmax, count);
-  balance_kernel<<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    dim3 g(kmax, std::min(MAX_GRID_Y, count - c0));
+    balance_kernel<<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 template <class T>

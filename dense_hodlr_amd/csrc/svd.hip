@@ -11,6 +11,8 @@
  */
 #include "svd.h"
 
+#include <algorithm>
+
 namespace hodlr {
 
 namespace {
@@ -125,16 +127,20 @@ void launch_svd_step(const SvdDesc* d, int count, int nwg, int step, unsigned* r
 void launch_svd_norms(const SvdDesc* d, int count, const int* off, int maxn, double* out,
                       hipStream_t s) {
   if (count == 0 || maxn == 0) return;
-  dim3 g(maxn, count);
-  svd_norms_kernel<<<g, TPB, 0, s>>>(d, off, out);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += 65535) {  // grid.y <= 65535
+    dim3 g(maxn, std::min(65535, count - c0));
+    svd_norms_kernel<<<g, TPB, 0, s>>>(d + c0, off + c0, out);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 void launch_col_copy(const ColCopyDesc* d, int count, hipStream_t s) {
   if (count == 0) return;
-  dim3 g(8, count);
-  col_copy_kernel<<<g, TPB, 0, s>>>(d);
-  HODLR_HIP_CHECK(hipGetLastError());
+  for (int c0 = 0; c0 < count; c0 += 65535) {
+    dim3 g(8, std::min(65535, count - c0));
+    col_copy_kernel<<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
 }
 
 }  // namespace hodlr
