@@ -267,7 +267,8 @@ def main():
         "aca_iterations": st["aca_iterations"],
         "predictor": {"blocks": st["pred_blocks"], "misses": st["pred_misses"],
                       "mean_abs_log_rank_error": st["pred_log_err"]},
-        "plain_fp64": {"value": plain, "compress_s": pst["compress_time"],
+        "step_walls_s": walls,
+        "plain_fp64": {"value": plain, "step_walls_s": pwalls, "compress_s": pst["compress_time"],
                        "factor_s": pst["factor_time"], "solve_and_refine_s": pst["solve_time"],
                        "residual": pst["last_residual"]},
         "speedup_vs_plain": plain / value,

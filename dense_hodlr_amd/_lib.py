@@ -30,7 +30,7 @@ FACTOR_SM, FACTOR_LU = 0, 1
 EXPORTS = [
     "hodlr_create_kernel", "hodlr_create_kernel_callback", "hodlr_create_dense", "hodlr_destroy",
     "hodlr_set_lr_method", "hodlr_set_tolerance", "hodlr_set_min_pivot", "hodlr_set_precision",
-    "hodlr_set_refinement", "hodlr_set_profiling", "hodlr_set_graphs", "hodlr_autotune", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
+    "hodlr_set_refinement", "hodlr_set_profiling", "hodlr_set_graphs", "hodlr_autotune", "hodlr_set_launch_shapes", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
     "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
     "hodlr_matvec", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
     "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
@@ -82,6 +82,7 @@ class Stats(C.Structure):
         ("tune_gemm_split_target", C.c_int),
         ("tune_lu_panel", C.c_int),
         ("tuned", C.c_int),
+        ("tune_gemm_mfma", C.c_int),
     ]
 
 
@@ -122,6 +123,7 @@ def lib():
         "hodlr_set_profiling": (C.c_int, [h, C.c_int]),
         "hodlr_set_graphs": (C.c_int, [h, C.c_int]),
         "hodlr_autotune": (C.c_int, [h]),
+        "hodlr_set_launch_shapes": (C.c_int, [h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
         "hodlr_compress": (C.c_int, [h]),
         "hodlr_factorize": (C.c_int, [h, C.c_int]),
         "hodlr_solve_sm": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),

@@ -239,6 +239,19 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
   return guard(h, [&] { h->eng->set_refinement(target, max_iters); });
 }
 
+int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_tile_m,
+                            int gemm_split_target, int lu_panel, int gemm_mfma) {
+  return guard(h, [&] {
+    hodlr::Tuning t;
+    t.aca_pt = aca_rows_per_thread;
+    t.gemm_bm = gemm_tile_m;
+    t.gemm_target = gemm_split_target;
+    t.lu_nb = lu_panel;
+    t.gemm_mfma = gemm_mfma;
+    h->eng->set_tuning(t);
+  });
+}
+
 int hodlr_autotune(hodlr_handle h) {
   return guard(h, [&] { h->eng->autotune(); });
 }
@@ -427,6 +440,7 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->tune_gemm_tile_m = tu.gemm_bm;
     out->tune_gemm_split_target = tu.gemm_target;
     out->tune_lu_panel = tu.lu_nb;
+    out->tune_gemm_mfma = tu.gemm_mfma;
     out->tuned = tu.tuned ? 1 : 0;
     out->graph_captures = s.graph_captures;
     out->graph_launches = s.graph_launches;

@@ -115,7 +115,12 @@ struct PermDesc {
   int chunk0; /* first workgroup (256 columns each) */
 };
 
-void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, hipStream_t s);
+/* Panel step of the batched blocked LU at columns [p0, p0 + nb); matrices of
+ * the batch have at most maxn rows.  lu_panel_width(maxn, nb) is the widest
+ * panel the register-resident kernel takes for that size (it falls back to
+ * the global-memory kernel beyond 4096 rows). */
+int lu_panel_width(int maxn, int nb);
+void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipStream_t s);
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,

@@ -194,6 +194,160 @@ __global__ void __launch_bounds__(TPB) lu_panel_kernel(const LuDesc* __restrict_
   }
 }
 
+/* The same panel step with the panel held in registers: thread t owns rows
+ * p0 + t + NT q (q < RPT) of the panel columns [p0, p0 + pw), pw <= PW.
+ * Per column: first-max pivot search over the owned rows (wave shuffles,
+ * then LDS), exchange of the two rows through LDS, broadcast of the pivot
+ * row, scaling and rank-1 update in registers.  The row interchanges of the
+ * columns outside the panel are applied afterwards, in the same order, one
+ * thread per column.  Used when the largest matrix of the batch has at most
+ * NT * RPT rows. */
+template <int NT, int RPT, int PW>
+__global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restrict__ d, int p0,
+                                                          int nb) {
+  constexpr int NW = NT / 64;
+  __shared__ double sv[NW];
+  __shared__ int si[NW];
+  __shared__ int piv_sh[PW];
+  __shared__ double xrow[2][PW];
+  __shared__ double prow[PW];
+  const LuDesc m = d[blockIdx.x];
+  const int n = m.n, ld = m.ld, t = threadIdx.x;
+  if (n <= p0) return;
+  double* A = m.A;
+  const int pw = min(min(nb, PW), n - p0);
+  double R[RPT][PW];
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = p0 + t + NT * q;
+#pragma unroll
+    for (int j = 0; j < PW; j++)
+      R[q][j] = (r < n && j < pw) ? A[(size_t)r + (size_t)(p0 + j) * ld] : 0.0;
+  }
+#pragma unroll
+  for (int c = 0; c < PW; c++) {
+    if (c < pw) {  // (uniform; a guard rather than a break keeps the loop unrolled)
+    const int cr = p0 + c;
+    // first maximum of |A(i, cr)|, i >= cr
+    double bv = -1.0;
+    int bi = n;
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = p0 + t + NT * q;
+      if (r >= cr && r < n) {
+        const double v = fabs(R[q][c]);
+        if (v > bv) {
+          bv = v;
+          bi = r;
+        }
+      }
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+      const double ov = __shfl_xor(bv, off);
+      const int oi = __shfl_xor(bi, off);
+      if (ov > bv || (ov == bv && oi < bi)) {
+        bv = ov;
+        bi = oi;
+      }
+    }
+    if ((t & 63) == 0) {
+      sv[t >> 6] = bv;
+      si[t >> 6] = bi;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double v = sv[0];
+      int ix = si[0];
+      for (int w = 1; w < NW; w++)
+        if (sv[w] > v || (sv[w] == v && si[w] < ix)) {
+          v = sv[w];
+          ix = si[w];
+        }
+      if (ix >= n) ix = cr;  // a column of NaNs has no maximum: keep the row
+      piv_sh[c] = ix;
+      m.piv[cr] = ix;
+    }
+    __syncthreads();
+    const int p = piv_sh[c];
+    // exchange rows cr and p of the panel
+    if (p != cr) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = p0 + t + NT * q;
+        if (r == cr || r == p) {
+#pragma unroll
+          for (int j = 0; j < PW; j++) xrow[r == cr ? 0 : 1][j] = R[q][j];
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = p0 + t + NT * q;
+        if (r == cr || r == p) {
+#pragma unroll
+          for (int j = 0; j < PW; j++) R[q][j] = xrow[r == cr ? 1 : 0][j];
+        }
+      }
+    }
+    // broadcast the pivot row
+#pragma unroll
+    for (int q = 0; q < RPT; q++)
+      if (p0 + t + NT * q == cr) {
+#pragma unroll
+        for (int j = 0; j < PW; j++) prow[j] = R[q][j];
+      }
+    __syncthreads();
+    const double pv = prow[c];
+    if (pv != 0.0) {
+      const double inv = 1.0 / pv;
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = p0 + t + NT * q;
+        if (r > cr && r < n) {
+          R[q][c] *= inv;
+#pragma unroll
+          for (int j = 0; j < PW; j++)
+            if (j > c) R[q][j] -= R[q][c] * prow[j];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int r = p0 + t + NT * q;
+        if (r > cr && r < n) {
+#pragma unroll
+          for (int j = 0; j < PW; j++)
+            if (j > c) R[q][j] -= R[q][c] * prow[j];
+        }
+      }
+    }
+    __syncthreads();
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < RPT; q++) {
+    const int r = p0 + t + NT * q;
+    if (r < n) {
+#pragma unroll
+      for (int j = 0; j < PW; j++)
+        if (j < pw) A[(size_t)r + (size_t)(p0 + j) * ld] = R[q][j];
+    }
+  }
+  // the same interchanges, in order, for the columns outside the panel
+  for (int j = t; j < n; j += NT) {
+    if (j >= p0 && j < p0 + pw) continue;
+    double* col = A + (size_t)j * ld;
+    for (int c = 0; c < pw; c++) {
+      const int p = piv_sh[c];
+      if (p != p0 + c) {
+        const double x = col[p0 + c];
+        col[p0 + c] = col[p];
+        col[p] = x;
+      }
+    }
+  }
+}
+
 /* Triangular solve of a pw x pw diagonal block (pw <= 64) on a chunk of 64
  * right-hand-side columns: unit lower (forward) or upper (backward).  One
  * wave per chunk, one column per lane, the block and the columns in LDS. */
@@ -441,9 +595,28 @@ void launch_assemble(const AssembleDesc* d, int count, int max_elems, const Entr
   }
 }
 
-void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, hipStream_t s) {
+int lu_panel_width(int maxn, int nb) {
+  if (maxn <= 512) return std::min(nb, 32);
+  if (maxn <= 1024) return std::min(nb, 16);
+  if (maxn <= 2048) return std::min(nb, 8);
+  if (maxn <= 4096) return std::min(nb, 4);
+  return nb;
+}
+
+void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipStream_t s) {
   if (count == 0) return;
-  lu_panel_kernel<<<count, TPB, 0, s>>>(d, p0, nb);
+  if (maxn <= 256 && nb <= 32)
+    lu_panel_reg_kernel<256, 1, 32><<<count, 256, 0, s>>>(d, p0, nb);
+  else if (maxn <= 512 && nb <= 32)
+    lu_panel_reg_kernel<512, 1, 32><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 1024 && nb <= 16)
+    lu_panel_reg_kernel<512, 2, 16><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 2048 && nb <= 8)
+    lu_panel_reg_kernel<512, 4, 8><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 4096 && nb <= 4)
+    lu_panel_reg_kernel<512, 8, 4><<<count, 512, 0, s>>>(d, p0, nb);
+  else
+    lu_panel_kernel<<<count, TPB, 0, s>>>(d, p0, nb);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

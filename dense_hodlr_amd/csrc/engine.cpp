@@ -83,6 +83,19 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   if (it != g_tune.end()) tune_ = it->second;
 }
 
+void Engine::set_tuning(const Tuning& t) {
+  if (t.aca_pt != 1 && t.aca_pt != 2 && t.aca_pt != 4)
+    throw std::invalid_argument("ACA rows per thread must be 1, 2 or 4");
+  if (t.gemm_bm != 0 && t.gemm_bm != 64 && t.gemm_bm != 128)
+    throw std::invalid_argument("GEMM tile rows must be 0, 64 or 128");
+  if (t.gemm_target < 1) throw std::invalid_argument("split-K target must be positive");
+  if (t.lu_nb < 1 || t.lu_nb > LU_NB) throw std::invalid_argument("LU panel width must be 1..64");
+  if (t.gemm_mfma != 0 && t.gemm_mfma != 1) throw std::invalid_argument("mfma must be 0 or 1");
+  tune_ = t;
+  tune_.tuned = false;
+  reset_attributes();
+}
+
 void Engine::autotune() {
   Tuning best = tune_;
   auto time_compress = [&](const Tuning& t) {
@@ -111,13 +124,16 @@ void Engine::autotune() {
   time_compress(best);
   bt = 1e300;
   Tuning fb = best;
-  const int shapes[][2] = {{0, 2048}, {0, 1024}, {0, 4096}, {64, 2048}, {128, 2048}};
+  // {tile rows (0: per launch), split-K target, matrix cores}
+  const int shapes[][3] = {{0, 2048, 0}, {0, 1024, 0}, {0, 4096, 0}, {64, 2048, 0},
+                           {128, 2048, 0}, {128, 2048, 1}, {128, 4096, 1}, {0, 4096, 1}};
   for (int nb : {64, 32})
     for (auto& sh : shapes) {
       Tuning t = best;
       t.lu_nb = nb;
       t.gemm_bm = sh[0];
       t.gemm_target = sh[1];
+      t.gemm_mfma = sh[2];
       const double v = std::min(time_factor(t), time_factor(t));
       if (v < bt) {
         bt = v;
@@ -366,7 +382,7 @@ void Engine::run_gemm_t(std::vector<GemmDescT<T>>& descs) {
   const GemmDescT<T>* d = arena_.push(descs, stream_);
   const int* ids = split_ids.empty() ? nullptr : arena_.push(split_ids, stream_);
   launch_gemm<T>(d, (int)descs.size(), ntiles, reinterpret_cast<T*>(ws_.p), ids,
-                 (int)split_ids.size(), bm, stream_);
+                 (int)split_ids.size(), bm, stream_, tune_.gemm_mfma != 0);
 }
 
 void Engine::run_gemm(std::vector<GemmDesc>& descs) { run_gemm_t<double>(descs); }
@@ -400,13 +416,14 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
   const LuDesc* dd = arena_.push(descs, stream_);
-  for (int p0 = 0; p0 < maxn; p0 += tune_.lu_nb) {
-    launch_lu_panel(dd, (int)descs.size(), p0, tune_.lu_nb, stream_);
+  const int nb = lu_panel_width(maxn, tune_.lu_nb);
+  for (int p0 = 0; p0 < maxn; p0 += nb) {
+    launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, stream_);
     std::vector<TrsmDesc> tr;
     std::vector<GemmDesc> g;
     int nwg = 0;
     for (auto& d : descs) {
-      const int pw = std::min(tune_.lu_nb, d.n - p0);
+      const int pw = std::min(nb, d.n - p0);
       const int rest = d.n - p0 - pw;
       if (pw <= 0 || rest <= 0) continue;
       TrsmDesc t{};

@@ -41,6 +41,7 @@ struct Tuning {
   int gemm_bm = 0;         // grouped GEMM tile rows (64, 128; 0: chosen per launch)
   int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
   int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
+  int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
   bool tuned = false;
 };
 
@@ -229,6 +230,7 @@ class Engine {
    * later handle of the same shape in this process. */
   void autotune();
   const Tuning& tuning() const { return tune_; }
+  void set_tuning(const Tuning& t);
   // replay FP64 solves (and FP64-preconditioned refinement steps) from
   // captured hipGraphs of the level sequence
   void set_graphs(bool on);

@@ -20,9 +20,10 @@ template <class T>
 size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, int& ntiles,
                  int target_tiles, int bm);
 
+/* mfma (FP64, bm 128 only): run the tiles on the matrix cores */
 template <class T>
 void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
-                 const int* d_split_ids, int nsplit, int bm, hipStream_t s);
+                 const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma = false);
 
 }  // namespace hodlr
 

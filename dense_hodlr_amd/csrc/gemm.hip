@@ -19,6 +19,7 @@
 #include "gemm.h"
 
 #include <algorithm>
+#include <type_traits>
 
 namespace hodlr {
 
@@ -160,6 +161,132 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   }
 }
 
+/* FP64 on the matrix cores: the same tiles, descriptors and split-K contract
+ * as gemm_kernel<double, 128, 64>, each wave computing a 64 x 32 sub-tile as
+ * 4 x 2 blocks of v_mfma_f64_16x16x4_f64.  Operand layout of that
+ * instruction (wave of 64): lane l holds A[l % 16][l / 16] of the 16 x 4
+ * A block and B[l / 16][l % 16] of the 4 x 16 B block; the 16 x 16 result
+ * is held as D[l / 16 + 4 i][l % 16] in register i = 0..3. */
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256)
+    gemm_mfma_f64_kernel(const GemmDescT<double>* __restrict__ descs, int count,
+                         double* __restrict__ ws) {
+  constexpr int BM = 128, BN = 64, NT = 256;
+  constexpr int A_PER = BM * BK / NT;
+  constexpr int B_PER = BN * BK / NT;
+  __shared__ double As[2][BK][BM + 2];
+  __shared__ double Bs[2][BK][BN + 2];
+  const GemmDescT<double> g = descs[find_problem(descs, count)];
+  const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
+  int tile = blockIdx.x - g.tile0;
+  const int split = tile / (tm * tn);
+  tile -= split * tm * tn;
+  const int m0 = (tile % tm) * BM, n0 = (tile / tm) * BN;
+  const int kchunk = ((g.K + g.ksplit - 1) / g.ksplit + BK - 1) / BK * BK;
+  const int kbeg = split * kchunk, kend = min(g.K, kbeg + kchunk);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int lr = lane & 15, lk = lane >> 4;
+
+  v4d acc[4][2];
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+
+  double ra[A_PER], rb[B_PER];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int r = 0; r < A_PER; r++) {
+      const int e = t + r * NT;
+      int mm, kk;
+      if (g.trans_a) {
+        kk = e % BK;
+        mm = e / BK;
+      } else {
+        mm = e % BM;
+        kk = e / BM;
+      }
+      const int gm = m0 + mm, gk = k0 + kk;
+      ra[r] = (gm < g.M && gk < kend)
+                  ? (g.trans_a ? g.A[(size_t)gk + (size_t)gm * g.lda] : g.A[(size_t)gm + (size_t)gk * g.lda])
+                  : 0.0;
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; r++) {
+      const int e = t + r * NT;
+      const int kk = e % BK, nn = e / BK;
+      const int gn = n0 + nn, gk = k0 + kk;
+      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : 0.0;
+    }
+  };
+  auto store = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < A_PER; r++) {
+      const int e = t + r * NT;
+      int mm, kk;
+      if (g.trans_a) {
+        kk = e % BK;
+        mm = e / BK;
+      } else {
+        mm = e % BM;
+        kk = e / BM;
+      }
+      As[buf][kk][mm] = ra[r];
+    }
+#pragma unroll
+    for (int r = 0; r < B_PER; r++) {
+      const int e = t + r * NT;
+      Bs[buf][e % BK][e / BK] = rb[r];
+    }
+  };
+
+  int buf = 0;
+  if (kbeg < kend) {
+    load(kbeg);
+    store(0);
+  }
+  __syncthreads();
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
+    const bool more = k0 + BK < kend;
+    if (more) load(k0 + BK);
+#pragma unroll
+    for (int kk = 0; kk < BK; kk += 4) {
+      double av[4], bv[2];
+#pragma unroll
+      for (int a = 0; a < 4; a++) av[a] = As[buf][kk + lk][wm + a * 16 + lr];
+#pragma unroll
+      for (int b = 0; b < 2; b++) bv[b] = Bs[buf][kk + lk][wn + b * 16 + lr];
+#pragma unroll
+      for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+    }
+    if (more) store(buf ^ 1);
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int a = 0; a < 4; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int gm = m0 + wm + a * 16 + lk + 4 * i, gn = n0 + wn + b * 16 + lr;
+        if (gm < g.M && gn < g.N) {
+          const double v = acc[a][b][i];
+          if (g.ksplit == 1) {
+            double* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+            *c = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *c;
+          } else {
+            ws[g.ws_off + (size_t)split * g.M * g.N + (size_t)gm + (size_t)gn * g.M] = v;
+          }
+        }
+      }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256)
     splitk_reduce_kernel(const GemmDescT<T>* __restrict__ descs, const int* __restrict__ ids,
@@ -229,8 +356,20 @@ size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, 
 
 template <class T>
 void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
-                 const int* d_split_ids, int nsplit, int bm, hipStream_t s) {
+                 const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma) {
   if (count == 0 || ntiles == 0) return;
+  if constexpr (std::is_same<T, double>::value) {
+    if (mfma && bm == 128) {
+      gemm_mfma_f64_kernel<<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+      HODLR_HIP_CHECK(hipGetLastError());
+      if (nsplit > 0) {
+        dim3 g(64, nsplit);
+        splitk_reduce_kernel<T><<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
+        HODLR_HIP_CHECK(hipGetLastError());
+      }
+      return;
+    }
+  }
   if (bm == 128)
     gemm_kernel<T, 128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
   else
@@ -248,8 +387,8 @@ template int gemm_tile_m<float>(const std::vector<GemmDescF>&);
 template size_t gemm_plan<double>(std::vector<GemmDesc>&, std::vector<int>&, int&, int, int);
 template size_t gemm_plan<float>(std::vector<GemmDescF>&, std::vector<int>&, int&, int, int);
 template void launch_gemm<double>(const GemmDesc*, int, int, double*, const int*, int, int,
-                                  hipStream_t);
+                                  hipStream_t, bool);
 template void launch_gemm<float>(const GemmDescF*, int, int, float*, const int*, int, int,
-                                 hipStream_t);
+                                 hipStream_t, bool);
 
 }  // namespace hodlr

@@ -112,6 +112,13 @@ class HODLR_Matrix:
         """Pick launch shapes by timing candidates on this problem."""
         check(lib().hodlr_autotune(self._h), self._h)
 
+    def set_launch_shapes(self, aca_rows_per_thread=2, gemm_tile_m=0, gemm_split_target=2048,
+                          lu_panel=64, gemm_mfma=False):
+        """Fix the launch shapes (see hodlr_set_launch_shapes)."""
+        check(lib().hodlr_set_launch_shapes(self._h, int(aca_rows_per_thread), int(gemm_tile_m),
+                                            int(gemm_split_target), int(lu_panel),
+                                            int(bool(gemm_mfma))), self._h)
+
     def set_graphs(self, on: bool = True):
         """Replay FP64 solves from captured hipGraphs."""
         check(lib().hodlr_set_graphs(self._h, int(bool(on))), self._h)
@@ -194,7 +201,8 @@ class HODLR_Matrix:
                     tuning=dict(aca_rows_per_thread=s.tune_aca_rows_per_thread,
                                 gemm_tile_m=s.tune_gemm_tile_m,
                                 gemm_split_target=s.tune_gemm_split_target,
-                                lu_panel=s.tune_lu_panel, tuned=bool(s.tuned)))
+                                lu_panel=s.tune_lu_panel, gemm_mfma=bool(s.tune_gemm_mfma),
+                                tuned=bool(s.tuned)))
 
     def get_LR_ComputationTime(self) -> float:
         return self.stats()["compress_time"]

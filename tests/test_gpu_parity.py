@@ -277,3 +277,29 @@ def test_svd_method_gauss1d_4096(gauss1d):
           f"rel diff to ACA solution {rel(xs, xa):.2e}", flush=True)
     assert rel(xs, xa) < 1e-7
     assert max(st["max_rank"]) <= max(g.stats()["max_rank"]) + 2
+
+
+@pytest.mark.parametrize("shapes", [
+    dict(aca_rows_per_thread=1, gemm_tile_m=64, gemm_split_target=1024, lu_panel=16),
+    dict(aca_rows_per_thread=4, gemm_tile_m=128, gemm_split_target=4096, lu_panel=32),
+    dict(aca_rows_per_thread=2, gemm_tile_m=128, gemm_split_target=2048, lu_panel=64,
+         gemm_mfma=True),
+    dict(aca_rows_per_thread=2, gemm_tile_m=128, gemm_split_target=64, lu_panel=64,
+         gemm_mfma=True),
+])
+def test_launch_shapes_match_oracle(oracle, shapes):
+    """Every launch-shape variant (ACA rows per thread, vector or matrix-core
+    FP64 GEMM tiles, split-K, LU panel width) gives the oracle's ranks and its
+    recSM / recLU solutions and log-determinant."""
+    _case(f"launch shapes {shapes}")
+    cfg = I.config_inputs("gauss1d_4096")
+    g, o = _pair(oracle, cfg["n"], cfg["points"], "gaussian", 0.1, 1e-2, 64, 1e-10)
+    g.set_launch_shapes(**shapes)
+    b = I.rhs_uniform(cfg["n"], 3, 5)
+    xs = g.recSM_Solve(b)
+    xl = g.recLU_Solve(b)
+    xo = o.recSM_Solve(b)
+    assert np.array_equal(g.nodes(), o.nodes())
+    assert rel(xs, xo) < 1e-10 and rel(xl, xo) < 1e-9
+    assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-11 * abs(o.logAbsDeterminant())
+    assert g.stats()["tuning"]["gemm_mfma"] == bool(shapes.get("gemm_mfma", False))
