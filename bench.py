@@ -53,6 +53,9 @@ def parse():
     p.add_argument("--no-roofline", action="store_true")
     p.add_argument("--no-learned-deploy", action="store_true")
     p.add_argument("--no-autotune", action="store_true")
+    p.add_argument("--launch-shapes", default=None,
+                   help="fix the launch shapes instead of autotuning: ACA rows per thread, "
+                        "GEMM tile rows, split-K target, LU panel, MFMA (e.g. 4,0,4096,32,1)")
     p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
     p.add_argument("--workload", default="gauss2d_262144", choices=["gauss2d_262144", "gp_batch_512"],
                    help="gp_batch_512: the batched Gaussian-process systems instead of the "
@@ -81,9 +84,14 @@ def workload(n, nrhs, seed=12345):
     return I.points_2d_kd(n, seed), I.rhs_normal(n, nrhs, seed + 1)
 
 
+SHAPES = None  # fixed launch shapes (--launch-shapes), applied to every handle
+
+
 def make_handle(pts, target, precision, max_steps):
     from dense_hodlr_amd import HODLR_Matrix
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+    if SHAPES:
+        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]))
     g.set_LRTolerance(1e-10)
     g.set_precision(precision)
     g.set_refinement(target, max_steps)
@@ -209,7 +217,12 @@ def main():
     # untimed: the autotuner picks launch shapes for this problem shape; every
     # later handle of the shape (both paths below) uses them
     tuning = None
-    if not a.no_autotune:
+    global SHAPES
+    shapes = [int(v) for v in a.launch_shapes.split(",")] if a.launch_shapes else None
+    SHAPES = shapes
+    if shapes is not None:
+        tuning = {"fixed": shapes}
+    if shapes is None and not a.no_autotune:
         g = make_handle(pts, a.target, "fp64_refined", 30)
         g.autotune()
         tuning = g.stats()["tuning"]
