@@ -287,6 +287,8 @@ def main():
         "speedup_vs_plain": plain / value,
         "tuning": tuning,
     }
+    from dense_hodlr_amd import memory_stats
+    out["device_memory_cache"] = memory_stats()
 
     if not a.no_mixed:
         from dense_hodlr_amd import HodlrError

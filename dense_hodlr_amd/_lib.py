@@ -39,6 +39,7 @@ EXPORTS = [
     "hodlr_rank_model_get_params", "hodlr_rank_model_set_params", "hodlr_rank_model_forward",
     "hodlr_rank_model_predict_bf16", "hodlr_rank_model_gradient", "hodlr_rank_model_train",
     "hodlr_block_features", "hodlr_set_rank_predictor",
+    "hodlr_memory_get_stats", "hodlr_memory_trim", "hodlr_memory_cache",
 ]
 
 
@@ -83,6 +84,17 @@ class Stats(C.Structure):
         ("tune_lu_panel", C.c_int),
         ("tuned", C.c_int),
         ("tune_gemm_mfma", C.c_int),
+    ]
+
+
+class MemoryStats(C.Structure):
+    _fields_ = [
+        ("live_bytes", C.c_longlong),
+        ("cached_bytes", C.c_longlong),
+        ("hits", C.c_longlong),
+        ("misses", C.c_longlong),
+        ("malloc_seconds", C.c_double),
+        ("free_seconds", C.c_double),
     ]
 
 
@@ -156,6 +168,9 @@ def lib():
                                              C.c_ulonglong, fp]),
         "hodlr_block_features": (C.c_int, [h, C.c_double, fp]),
         "hodlr_set_rank_predictor": (C.c_int, [h, h, C.c_double]),
+        "hodlr_memory_get_stats": (C.c_int, [C.POINTER(MemoryStats)]),
+        "hodlr_memory_trim": (C.c_int, []),
+        "hodlr_memory_cache": (C.c_int, [C.c_int]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

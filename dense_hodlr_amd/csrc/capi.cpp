@@ -506,4 +506,37 @@ const char* hodlr_last_error(hodlr_handle h) {
 
 const char* hodlr_version(void) { return "dense_hodlr_amd 0.1 (gfx950)"; }
 
+int hodlr_memory_get_stats(hodlr_memory_stats* out) {
+  if (!out) return fail(nullptr, HODLR_ERR_ARG, "null stats pointer");
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  try {
+    const hodlr::PoolStats p = hodlr::pool_stats();
+    out->live_bytes = p.live_bytes;
+    out->cached_bytes = p.cached_bytes;
+    out->hits = p.hits;
+    out->misses = p.misses;
+    out->malloc_seconds = p.malloc_s;
+    out->free_seconds = p.free_s;
+  } catch (const std::exception& e) {
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  }
+  return HODLR_OK;
+}
+
+int hodlr_memory_trim(void) {
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  try {
+    hodlr::pool_trim();
+  } catch (const std::exception& e) {
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  }
+  return HODLR_OK;
+}
+
+int hodlr_memory_cache(int on) {
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  hodlr::pool_set_enabled(on != 0);
+  return HODLR_OK;
+}
+
 }  // extern "C"

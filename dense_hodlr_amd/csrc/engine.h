@@ -30,6 +30,7 @@
 #include "ddres.h"
 #include "dense.h"
 #include "gemm.h"
+#include "pool.h"
 
 namespace hodlr {
 
@@ -69,18 +70,11 @@ struct DBuf {
   ~DBuf() { release(); }
   void alloc(size_t cnt) {
     release();
-    if (cnt) {
-      hipError_t e = hipMalloc((void**)&p, cnt * sizeof(T));
-      if (e != hipSuccess) {
-        p = nullptr;
-        throw HipError(std::string("hipMalloc of ") + std::to_string(cnt * sizeof(T)) +
-                       " bytes failed: " + hipGetErrorString(e));
-      }
-    }
+    if (cnt) p = static_cast<T*>(pool_alloc(cnt * sizeof(T)));
     n = cnt;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    pool_release(p);
     p = nullptr;
     n = 0;
   }

@@ -306,6 +306,31 @@ const char* hodlr_last_error(hodlr_handle h);
 /* Library version string. */
 const char* hodlr_version(void);
 
+/* ---- device memory cache ----------------------------------------------- */
+
+/* Device blocks released by handles (destroyed, grown, refactored) are kept
+ * per device and reused for later requests they fit, so repeated handles of
+ * one shape stop calling hipMalloc/hipFree; the cache holds at most half of
+ * the device's memory and is given back when an allocation fails.
+ * HODLR_NO_MEMORY_CACHE in the environment starts with the cache off.  No
+ * reference counterpart. */
+typedef struct hodlr_memory_stats {
+  long long live_bytes;   /* in use by handles */
+  long long cached_bytes; /* released, kept for reuse */
+  long long hits;         /* requests served from the cache */
+  long long misses;       /* requests that called hipMalloc */
+  double malloc_seconds;  /* host time spent in hipMalloc */
+  double free_seconds;    /* host time spent in hipFree */
+} hodlr_memory_stats;
+
+/* Counters of the current device. */
+int hodlr_memory_get_stats(hodlr_memory_stats* out);
+/* Frees every cached block of the current device (after it is idle). */
+int hodlr_memory_trim(void);
+/* on 0: releases free at once and the caches of all devices are emptied;
+ * on 1: releases are cached again. */
+int hodlr_memory_cache(int on);
+
 #ifdef __cplusplus
 }
 #endif
