@@ -56,11 +56,13 @@ int dd_proj_wgs(const DdProjDesc& d);
 int dd_reduce_wgs(const DdProjDesc& d);
 int dd_apply_wgs(const DdApplyDesc& d);
 
-void launch_dd_proj(const DdProjDesc* d, int count, int nwg, double* part_hi, double* part_lo,
-                    hipStream_t s);
+/* nrhs: the right-hand side count of every problem of the launch (selects
+ * the tile shape, as dd_proj_wgs / dd_apply_wgs do) */
+void launch_dd_proj(const DdProjDesc* d, int count, int nwg, int nrhs, double* part_hi,
+                    double* part_lo, hipStream_t s);
 void launch_dd_reduce(const DdProjDesc* d, int count, int nwg, const double* part_hi,
                       const double* part_lo, double* t_hi, double* t_lo, hipStream_t s);
-void launch_dd_apply(const DdApplyDesc* d, int count, int nwg, hipStream_t s);
+void launch_dd_apply(const DdApplyDesc* d, int count, int nwg, int nrhs, hipStream_t s);
 /* R = B - (Y_hi + Y_lo), rounded once (n x ncols) */
 void launch_dd_residual(const double* B, int ldb, const double* y_hi, const double* y_lo, int ldy,
                         double* R, int ldr, int n, int ncols, hipStream_t s);

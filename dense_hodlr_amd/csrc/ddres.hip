@@ -5,13 +5,17 @@
  * TwoProd via FMA gives p + e = a * b exactly.  A Dot2 accumulator keeps the
  * running sum in hi and the accumulated rounding errors in lo.
  *
- *   projection  T = V^T X per node half and level: 16 x 16 (c, j) output tile
- *               per workgroup over a chunk of DD_CHUNK rows, operands staged
- *               through LDS 64 rows at a time; chunk partials reduced in
- *               chunk order by a second kernel (deterministic).
+ *   projection  T = V^T X per node half and level: a 64 x 64 (c, j) output
+ *               tile per workgroup (64 x 16 or 256 x 4 for few right-hand
+ *               sides), 4 x 4 (or 4 x 1) outputs per thread, over a chunk of
+ *               DD_CHUNK rows; operands staged through LDS 32 rows at a time
+ *               with the next rows' loads in flight.  X is exact, so each
+ *               term is TwoProd + TwoSum.  Chunk partials are reduced in chunk
+ *               order by a second kernel (deterministic).
  *   apply       Y += A T (leaves: A = leaf block, T = X; low rank: A = U,
  *               T = projection): one row per thread, 16 right-hand sides per
- *               workgroup, T staged through LDS 128 terms at a time.
+ *               workgroup (4 for fewer), T staged through LDS 128 terms at a
+ *               time.
  */
 #include "ddres.h"
 
@@ -28,18 +32,6 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
   e = (a - (s - bb)) + (b - bb);
 }
 
-/* (sh, sl) += a * (th, tl) */
-__device__ __forceinline__ void dot2_step(double a, double th, double tl, double& sh,
-                                          double& sl) {
-#pragma clang fp contract(off)
-  const double p = a * th;
-  const double pe = __builtin_fma(a, th, -p);
-  double s, q;
-  two_sum(sh, p, s, q);
-  sh = s;
-  sl += (q + pe) + a * tl;
-}
-
 template <class D>
 __device__ __forceinline__ int find_desc(const D* __restrict__ d, int count, int key,
                                          int D::*field) {
@@ -54,42 +46,113 @@ __device__ __forceinline__ int find_desc(const D* __restrict__ d, int count, int
   return lo;
 }
 
-constexpr int PT = 16;  // projection tile: PT columns c x PT right-hand sides j
-constexpr int PR = 64;  // rows staged per step
+/* (sh, sl) += a * b, b exact (no low part) */
+__device__ __forceinline__ void dot2_step_exact(double a, double b, double& sh, double& sl) {
+#pragma clang fp contract(off)
+  const double p = a * b;
+  const double pe = __builtin_fma(a, b, -p);
+  double s, q;
+  two_sum(sh, p, s, q);
+  sh = s;
+  sl += q + pe;
+}
 
+/* Projection tile shapes: TC threads along c, TPB / TC along j; each thread
+ * owns CA x JB outputs (c = c0 + tc + TC a, j = j0 + tj + (TPB / TC) b), so a
+ * row step reads CA + JB operands from LDS for CA * JB products.  PR rows are
+ * staged per step, the next step's global loads in flight meanwhile. */
+template <int TC, int CA, int JB>
+struct ProjShape {
+  static constexpr int TJ = TPB / TC;
+  static constexpr int CT = TC * CA;  // c tile
+  static constexpr int JT = TJ * JB;  // j tile
+  static constexpr int PR = CT + JT > 128 ? 16 : 32;
+};
+using ProjWide = ProjShape<16, 4, 4>;   // 64 x 64, many right-hand sides
+using ProjMid = ProjShape<16, 4, 1>;    // 64 x 16
+using ProjNarrow = ProjShape<64, 4, 1>; // 256 x 4, up to 4 right-hand sides
+
+template <int TC, int CA, int JB>
 __global__ void __launch_bounds__(TPB)
     dd_proj_kernel(const DdProjDesc* __restrict__ descs, int count, double* __restrict__ ph,
                    double* __restrict__ pl) {
-  __shared__ double Vs[PR][PT + 1];
-  __shared__ double Xs[PR][PT + 1];
+  using S = ProjShape<TC, CA, JB>;
+  constexpr int CT = S::CT, JT = S::JT, PR = S::PR, TJ = S::TJ;
+  constexpr int VL = PR * CT / TPB, XL = (PR * JT + TPB - 1) / TPB;
+  __shared__ double Vs[PR][CT + 1];
+  __shared__ double Xs[PR][JT + 1];
   const DdProjDesc d = descs[find_desc(descs, count, (int)blockIdx.x, &DdProjDesc::wg0)];
-  const int nct = (d.k + PT - 1) / PT, njt = (d.nrhs + PT - 1) / PT;
+  const int nct = (d.k + CT - 1) / CT, njt = (d.nrhs + JT - 1) / JT;
   int w = (int)blockIdx.x - d.wg0;
   const int chunk = w / (nct * njt);
   w -= chunk * nct * njt;
-  const int c0 = (w % nct) * PT, j0 = (w / nct) * PT;
-  const int t = threadIdx.x, c = t % PT, j = t / PT;
+  const int c0 = (w % nct) * CT, j0 = (w / nct) * JT;
+  const int t = threadIdx.x, tc = t % TC, tj = t / TC;
   const int rbeg = chunk * DD_CHUNK, rend = min(d.rows, rbeg + DD_CHUNK);
-  double sh = 0.0, sl = 0.0;
+  double sh[CA][JB], sl[CA][JB];
+#pragma unroll
+  for (int a = 0; a < CA; a++)
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+      sh[a][b] = 0.0;
+      sl[a][b] = 0.0;
+    }
+  double rv[VL], rx[XL];
+  auto load = [&](int r0) {
+#pragma unroll
+    for (int q = 0; q < VL; q++) {
+      const int e = t + q * TPB;
+      const int rr = e % PR, cc = e / PR, gr = r0 + rr;
+      rv[q] = (gr < rend && c0 + cc < d.k) ? d.V[(size_t)gr + (size_t)(c0 + cc) * d.ldv] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < XL; q++) {
+      const int e = t + q * TPB;
+      const int rr = e % PR, cc = e / PR, gr = r0 + rr;
+      rx[q] = (e < PR * JT && gr < rend && j0 + cc < d.nrhs)
+                  ? d.X[(size_t)gr + (size_t)(j0 + cc) * d.ldx]
+                  : 0.0;
+    }
+  };
+  if (rbeg < rend) load(rbeg);
   for (int r0 = rbeg; r0 < rend; r0 += PR) {
 #pragma unroll
-    for (int q = 0; q < PR * PT / TPB; q++) {
+    for (int q = 0; q < VL; q++) {
       const int e = t + q * TPB;
-      const int rr = e % PR, cc = e / PR;
-      const int gr = r0 + rr;
-      Vs[rr][cc] = (gr < rend && c0 + cc < d.k) ? d.V[(size_t)gr + (size_t)(c0 + cc) * d.ldv] : 0.0;
-      Xs[rr][cc] = (gr < rend && j0 + cc < d.nrhs) ? d.X[(size_t)gr + (size_t)(j0 + cc) * d.ldx] : 0.0;
+      Vs[e % PR][e / PR] = rv[q];
+    }
+#pragma unroll
+    for (int q = 0; q < XL; q++) {
+      const int e = t + q * TPB;
+      if (e < PR * JT) Xs[e % PR][e / PR] = rx[q];
     }
     __syncthreads();
+    if (r0 + PR < rend) load(r0 + PR);
     const int nr = min(PR, rend - r0);
-    for (int rr = 0; rr < nr; rr++) dot2_step(Vs[rr][c], Xs[rr][j], 0.0, sh, sl);
+    for (int rr = 0; rr < nr; rr++) {
+      double va[CA], xb[JB];
+#pragma unroll
+      for (int a = 0; a < CA; a++) va[a] = Vs[rr][tc + TC * a];
+#pragma unroll
+      for (int b = 0; b < JB; b++) xb[b] = Xs[rr][tj + TJ * b];
+#pragma unroll
+      for (int a = 0; a < CA; a++)
+#pragma unroll
+        for (int b = 0; b < JB; b++) dot2_step_exact(va[a], xb[b], sh[a][b], sl[a][b]);
+    }
     __syncthreads();
   }
-  if (c0 + c < d.k && j0 + j < d.nrhs) {
-    const size_t o = (size_t)d.part_off + ((size_t)chunk * d.nrhs + (j0 + j)) * d.k + (c0 + c);
-    ph[o] = sh;
-    pl[o] = sl;
-  }
+#pragma unroll
+  for (int a = 0; a < CA; a++)
+#pragma unroll
+    for (int b = 0; b < JB; b++) {
+      const int c = c0 + tc + TC * a, j = j0 + tj + TJ * b;
+      if (c < d.k && j < d.nrhs) {
+        const size_t o = (size_t)d.part_off + ((size_t)chunk * d.nrhs + j) * d.k + c;
+        ph[o] = sh[a][b];
+        pl[o] = sl[a][b];
+      }
+    }
 }
 
 __global__ void __launch_bounds__(TPB)
@@ -114,9 +177,26 @@ __global__ void __launch_bounds__(TPB)
   tl[d.t_off + e] = l;
 }
 
-constexpr int AJ = 16;   // right-hand sides per apply workgroup
 constexpr int AK = 128;  // terms staged per step
 
+/* (sh, sl) += a * (th, tl): the product a * th exactly (TwoProd), its sum
+ * with sh exactly (TwoSum); the low-order terms, a * tl included, are
+ * accumulated in sl with a fused multiply-add (their own rounding is far
+ * below the pair's precision). */
+__device__ __forceinline__ void dot2_step_lo(double a, double th, double tl, double& sh,
+                                             double& sl) {
+  double p, pe, s, q;
+  {
+#pragma clang fp contract(off)
+    p = a * th;
+    pe = __builtin_fma(a, th, -p);
+    two_sum(sh, p, s, q);
+  }
+  sh = s;
+  sl = __builtin_fma(a, tl, sl + (q + pe));
+}
+
+template <int AJ>
 __global__ void __launch_bounds__(TPB)
     dd_apply_kernel(const DdApplyDesc* __restrict__ descs, int count) {
   __shared__ double Th[AK][AJ];
@@ -136,13 +216,15 @@ __global__ void __launch_bounds__(TPB)
   for (int c0 = 0; c0 < d.k; c0 += AK) {
     const int nc = min(AK, d.k - c0);
 #pragma unroll
-    for (int q = 0; q < AK * AJ / TPB; q++) {
+    for (int q = 0; q < (AK * AJ + TPB - 1) / TPB; q++) {
       const int e = t + q * TPB;
-      const int cc = e % AK, jj = e / AK;
-      const bool ok = cc < nc && jj < nj;
-      const size_t o = (size_t)(c0 + cc) + (size_t)(j0 + jj) * d.ldt;
-      Th[cc][jj] = ok ? d.t_hi[o] : 0.0;
-      Tl[cc][jj] = (ok && d.t_lo) ? d.t_lo[o] : 0.0;
+      if (e < AK * AJ) {
+        const int cc = e % AK, jj = e / AK;
+        const bool ok = cc < nc && jj < nj;
+        const size_t o = (size_t)(c0 + cc) + (size_t)(j0 + jj) * d.ldt;
+        Th[cc][jj] = ok ? d.t_hi[o] : 0.0;
+        Tl[cc][jj] = (ok && d.t_lo) ? d.t_lo[o] : 0.0;
+      }
     }
     __syncthreads();
     if (row < d.rows) {
@@ -150,7 +232,7 @@ __global__ void __launch_bounds__(TPB)
       for (int cc = 0; cc < nc; cc++) {
         const double av = a[(size_t)cc * d.lda];
 #pragma unroll
-        for (int jj = 0; jj < AJ; jj++) dot2_step(av, Th[cc][jj], Tl[cc][jj], sh[jj], sl[jj]);
+        for (int jj = 0; jj < AJ; jj++) dot2_step_lo(av, Th[cc][jj], Tl[cc][jj], sh[jj], sl[jj]);
       }
     }
     __syncthreads();
@@ -190,18 +272,38 @@ __global__ void __launch_bounds__(TPB)
 
 }  // namespace
 
+namespace {
+// tile shape by right-hand side count (uniform over one launch)
+template <class F>
+auto with_proj_shape(int nrhs, F&& f) {
+  if (nrhs > 16) return f(ProjWide{});
+  if (nrhs > 4) return f(ProjMid{});
+  return f(ProjNarrow{});
+}
+int apply_cols(int nrhs) { return nrhs > 4 ? 16 : 4; }
+}  // namespace
+
 int dd_proj_wgs(const DdProjDesc& d) {
-  return d.nchunks * ((d.k + PT - 1) / PT) * ((d.nrhs + PT - 1) / PT);
+  return with_proj_shape(d.nrhs, [&](auto sh) {
+    using S = decltype(sh);
+    return d.nchunks * ((d.k + S::CT - 1) / S::CT) * ((d.nrhs + S::JT - 1) / S::JT);
+  });
 }
 int dd_reduce_wgs(const DdProjDesc& d) { return (d.k * d.nrhs + TPB - 1) / TPB; }
 int dd_apply_wgs(const DdApplyDesc& d) {
-  return ((d.rows + TPB - 1) / TPB) * ((d.nrhs + AJ - 1) / AJ);
+  const int aj = apply_cols(d.nrhs);
+  return ((d.rows + TPB - 1) / TPB) * ((d.nrhs + aj - 1) / aj);
 }
 
-void launch_dd_proj(const DdProjDesc* d, int count, int nwg, double* part_hi, double* part_lo,
-                    hipStream_t s) {
+void launch_dd_proj(const DdProjDesc* d, int count, int nwg, int nrhs, double* part_hi,
+                    double* part_lo, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
-  dd_proj_kernel<<<nwg, TPB, 0, s>>>(d, count, part_hi, part_lo);
+  if (nrhs > 16)
+    dd_proj_kernel<16, 4, 4><<<nwg, TPB, 0, s>>>(d, count, part_hi, part_lo);
+  else if (nrhs > 4)
+    dd_proj_kernel<16, 4, 1><<<nwg, TPB, 0, s>>>(d, count, part_hi, part_lo);
+  else
+    dd_proj_kernel<64, 4, 1><<<nwg, TPB, 0, s>>>(d, count, part_hi, part_lo);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
@@ -212,9 +314,13 @@ void launch_dd_reduce(const DdProjDesc* d, int count, int nwg, const double* par
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-void launch_dd_apply(const DdApplyDesc* d, int count, int nwg, hipStream_t s) {
+void launch_dd_apply(const DdApplyDesc* d, int count, int nwg, int nrhs, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
-  dd_apply_kernel<<<nwg, TPB, 0, s>>>(d, count);
+  const int aj = apply_cols(nrhs);
+  if (aj == 16)
+    dd_apply_kernel<16><<<nwg, TPB, 0, s>>>(d, count);
+  else
+    dd_apply_kernel<4><<<nwg, TPB, 0, s>>>(d, count);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

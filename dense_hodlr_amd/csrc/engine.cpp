@@ -2117,7 +2117,7 @@ void Engine::residual_dd(const double* dB, int ldb, const double* dX, int ldx, i
   double* tl = ddt_.p + std::max<long long>(toff, 1);
   if (!pd.empty()) {
     const DdProjDesc* dpd = arena_.push(pd, stream_);
-    launch_dd_proj(dpd, (int)pd.size(), pwg, ph, pl, stream_);
+    launch_dd_proj(dpd, (int)pd.size(), pwg, nrhs, ph, pl, stream_);
     launch_dd_reduce(dpd, (int)pd.size(), rwg, ph, pl, th, tl, stream_);
   }
   // leaves: Y = L X
@@ -2142,7 +2142,7 @@ void Engine::residual_dd(const double* dB, int ldb, const double* dX, int ldx, i
       wg += dd_apply_wgs(a);
       ad.push_back(a);
     }
-    launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, stream_);
+    launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, nrhs, stream_);
   }
   // low-rank updates, level by level (levels overlap in rows)
   for (int d = 0; d < nlev_; d++) {
@@ -2171,7 +2171,7 @@ void Engine::residual_dd(const double* dB, int ldb, const double* dX, int ldx, i
       wg += dd_apply_wgs(a);
       ad.push_back(a);
     }
-    if (!ad.empty()) launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, stream_);
+    if (!ad.empty()) launch_dd_apply(arena_.push(ad, stream_), (int)ad.size(), wg, nrhs, stream_);
   }
   launch_dd_residual(dB, ldb, yh, yl, ld, dR, ldr, ntot_, nrhs, stream_);
   HODLR_HIP_CHECK(hipStreamSynchronize(stream_));

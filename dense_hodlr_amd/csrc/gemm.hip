@@ -27,12 +27,22 @@ namespace {
 
 constexpr int BK = 16;
 
+/* Workgroups go to the 8 XCDs of the chip round-robin (workgroup i to XCD
+ * i % 8), each XCD with its own L2.  Tiles are renumbered so that every XCD
+ * runs one contiguous range of them: neighbouring tiles, which share operand
+ * columns and rows, then meet in the same L2.  A bijection of [0, n). */
+__device__ __forceinline__ int xcd_tile() {
+  const int n = (int)gridDim.x, i = (int)blockIdx.x;
+  const int q = n / 8, r = n % 8, x = i % 8, p = i / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + p;
+}
+
 template <class D>
-__device__ __forceinline__ int find_problem(const D* __restrict__ descs, int count) {
+__device__ __forceinline__ int find_problem(const D* __restrict__ descs, int count, int tile) {
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (descs[mid].tile0 <= (int)blockIdx.x)
+    if (descs[mid].tile0 <= tile)
       lo = mid;
     else
       hi = mid - 1;
@@ -50,9 +60,10 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   constexpr int B_PER = BN * BK / NT;
   __shared__ T As[2][BK][BM + 1];
   __shared__ T Bs[2][BK][BN + 1];
-  const GemmDescT<T> g = descs[find_problem(descs, count)];
+  const int gtile = xcd_tile();
+  const GemmDescT<T> g = descs[find_problem(descs, count, gtile)];
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
-  int tile = blockIdx.x - g.tile0;
+  int tile = gtile - g.tile0;
   const int split = tile / (tm * tn);
   tile -= split * tm * tn;
   const int m0 = (tile % tm) * BM, n0 = (tile / tm) * BN;
@@ -177,9 +188,10 @@ __global__ void __launch_bounds__(256)
   constexpr int B_PER = BN * BK / NT;
   __shared__ double As[2][BK][BM + 2];
   __shared__ double Bs[2][BK][BN + 2];
-  const GemmDescT<double> g = descs[find_problem(descs, count)];
+  const int gtile = xcd_tile();
+  const GemmDescT<double> g = descs[find_problem(descs, count, gtile)];
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
-  int tile = blockIdx.x - g.tile0;
+  int tile = gtile - g.tile0;
   const int split = tile / (tm * tn);
   tile -= split * tm * tn;
   const int m0 = (tile % tm) * BM, n0 = (tile / tm) * BN;
@@ -189,11 +201,22 @@ __global__ void __launch_bounds__(256)
   const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
+  // C += A B and C -= A B (the factorization's updates) start the sums from
+  // +-C, so the loads of C overlap the first operand slices instead of
+  // following the last one; alpha = +-1 keeps every step exact but the sums
+  const bool cinit = g.ksplit == 1 && g.beta == 1.0 && (g.alpha == 1.0 || g.alpha == -1.0);
   v4d acc[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
-    for (int b = 0; b < 2; b++) acc[a][b] = v4d{0.0, 0.0, 0.0, 0.0};
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int gm = m0 + wm + a * 16 + lk + 4 * i, gn = n0 + wn + b * 16 + lr;
+        acc[a][b][i] = (cinit && gm < g.M && gn < g.N)
+                           ? g.alpha * g.C[(size_t)gm + (size_t)gn * g.ldc]
+                           : 0.0;
+      }
 
   double ra[A_PER], rb[B_PER];
   auto load = [&](int k0) {
@@ -277,7 +300,9 @@ __global__ void __launch_bounds__(256)
         const int gm = m0 + wm + a * 16 + lk + 4 * i, gn = n0 + wn + b * 16 + lr;
         if (gm < g.M && gn < g.N) {
           const double v = acc[a][b][i];
-          if (g.ksplit == 1) {
+          if (cinit) {
+            g.C[(size_t)gm + (size_t)gn * g.ldc] = g.alpha * v;
+          } else if (g.ksplit == 1) {
             double* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
             *c = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *c;
           } else {
