@@ -123,6 +123,13 @@ int lu_panel_width(int maxn, int nb);
 void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipStream_t s);
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
+/* Whole solves of systems of at most LU_SOLVE_SMALL_MAX rows in one launch
+ * (interchanges, both substitutions); SolveDesc::chunk0 numbers the
+ * workgroups, lu_solve_small_wgs(ncols) per problem.  Returns false, and
+ * launches nothing, when maxn exceeds the limit. */
+constexpr int LU_SOLVE_SMALL_MAX = 2048;
+int lu_solve_small_wgs(int ncols);
+bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s);
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);

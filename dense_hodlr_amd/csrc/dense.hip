@@ -580,6 +580,132 @@ int grid_x(size_t elems, int ndesc) {
 }
 
 /* grid.y is limited to 65535: descriptors are launched in chunks */
+/* Whole LU solve A X = B (row interchanges, unit lower, upper) of small
+ * systems in one launch: one workgroup per problem and SOLVE_CB columns of B.
+ * Thread t keeps rows t + 256 r (r < R) of its columns in registers for the
+ * whole solve; each substitution step the owner of the pivot row publishes
+ * x_l through LDS (double-buffered by step parity, one barrier per step) and
+ * every thread updates its rows with column l of the factor, whose entries
+ * for the next step are loaded during the current one. */
+constexpr int SOLVE_CB = 8;
+
+template <int R>
+__global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __restrict__ d,
+                                                             int count) {
+  __shared__ int perm[R * TPB];
+  __shared__ double xs[2][SOLVE_CB];
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SolveDesc p = d[lo];
+  const int n = p.n, t = threadIdx.x;
+  const int c0 = ((int)blockIdx.x - p.chunk0) * SOLVE_CB;
+  const int nc = min(SOLVE_CB, p.ncols - c0);
+  // the interchanges piv[0..n), applied in order, as one permutation
+  for (int i = t; i < n; i += TPB) perm[i] = i;
+  __syncthreads();
+  if (t == 0)
+    for (int k = 0; k < n; k++) {
+      const int q = p.piv[k];
+      if (q != k) {
+        const int x = perm[k];
+        perm[k] = perm[q];
+        perm[q] = x;
+      }
+    }
+  __syncthreads();
+  double x[R][SOLVE_CB];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+#pragma unroll
+    for (int c = 0; c < SOLVE_CB; c++)
+      x[r][c] = (i < n && c < nc) ? p.B[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
+  }
+  const double* LU = p.LU;
+  const size_t ld = (size_t)p.ld_lu;
+  // forward: unit lower triangle
+  double f[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+    f[r] = (i > 0 && i < n) ? LU[(size_t)i] : 0.0;  // column 0
+  }
+  for (int l = 0; l < n; l++) {
+    const int own = l / TPB;
+    if (t == l % TPB) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (r == own)
+#pragma unroll
+          for (int c = 0; c < SOLVE_CB; c++) xs[l & 1][c] = x[r][c];
+    }
+    double fn[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {  // column l + 1, rows below the diagonal
+      const int i = t + TPB * r;
+      fn[r] = (l + 1 < n && i > l + 1 && i < n) ? LU[(size_t)i + (size_t)(l + 1) * ld] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = t + TPB * r;
+      if (i > l && i < n)
+#pragma unroll
+        for (int c = 0; c < SOLVE_CB; c++) x[r][c] -= f[r] * xs[l & 1][c];
+      f[r] = fn[r];
+    }
+  }
+  __syncthreads();
+  // backward: upper triangle with its diagonal
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+    f[r] = (n > 0 && i <= n - 1 && i < n) ? LU[(size_t)i + (size_t)(n - 1) * ld] : 0.0;
+  }
+  for (int l = n - 1; l >= 0; l--) {
+    const int own = l / TPB;
+    if (t == l % TPB) {
+#pragma unroll
+      for (int r = 0; r < R; r++)
+        if (r == own)
+#pragma unroll
+          for (int c = 0; c < SOLVE_CB; c++) {
+            x[r][c] = x[r][c] / f[r];
+            xs[l & 1][c] = x[r][c];
+          }
+    }
+    double fn[R];
+#pragma unroll
+    for (int r = 0; r < R; r++) {  // column l - 1, rows on and above the diagonal
+      const int i = t + TPB * r;
+      fn[r] = (l >= 1 && i <= l - 1) ? LU[(size_t)i + (size_t)(l - 1) * ld] : 0.0;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = t + TPB * r;
+      if (i < l)
+#pragma unroll
+        for (int c = 0; c < SOLVE_CB; c++) x[r][c] -= f[r] * xs[l & 1][c];
+      f[r] = fn[r];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < SOLVE_CB; c++)
+        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+  }
+}
+
 constexpr int MAX_GRID_Y = 65535;
 
 }  // namespace
@@ -624,6 +750,23 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
   trsm_kernel<<<nwg, 64, 0, s>>>(d, count);
   HODLR_HIP_CHECK(hipGetLastError());
+}
+
+int lu_solve_small_wgs(int ncols) { return (ncols + SOLVE_CB - 1) / SOLVE_CB; }
+
+bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s) {
+  if (maxn > LU_SOLVE_SMALL_MAX) return false;
+  if (count == 0 || nwg == 0) return true;
+  if (maxn <= TPB)
+    lu_solve_small_kernel<1><<<nwg, TPB, 0, s>>>(d, count);
+  else if (maxn <= 2 * TPB)
+    lu_solve_small_kernel<2><<<nwg, TPB, 0, s>>>(d, count);
+  else if (maxn <= 4 * TPB)
+    lu_solve_small_kernel<4><<<nwg, TPB, 0, s>>>(d, count);
+  else
+    lu_solve_small_kernel<8><<<nwg, TPB, 0, s>>>(d, count);
+  HODLR_HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s) {

@@ -454,6 +454,16 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
 void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
   if (descs.empty()) return;
   int maxn = 0;
+  for (auto& d : descs) maxn = std::max(maxn, d.n);
+  if (maxn <= LU_SOLVE_SMALL_MAX) {  // every system in one launch
+    int nwg = 0;
+    for (auto& d : descs) {
+      d.chunk0 = nwg;
+      nwg += lu_solve_small_wgs(d.ncols);
+    }
+    launch_lu_solve_small(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, stream_);
+    return;
+  }
   {
     std::vector<PermDesc> pd;
     int nwg = 0;
