@@ -593,6 +593,7 @@ template <int R>
 __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __restrict__ d,
                                                              int count) {
   __shared__ int perm[R * TPB];
+  __shared__ int pv[R * TPB];
   __shared__ double xs[2][SOLVE_CB];
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -606,12 +607,16 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
   const int n = p.n, t = threadIdx.x;
   const int c0 = ((int)blockIdx.x - p.chunk0) * SOLVE_CB;
   const int nc = min(SOLVE_CB, p.ncols - c0);
-  // the interchanges piv[0..n), applied in order, as one permutation
-  for (int i = t; i < n; i += TPB) perm[i] = i;
+  // the interchanges piv[0..n), applied in order, as one permutation (read
+  // by all threads, replayed from LDS by one)
+  for (int i = t; i < n; i += TPB) {
+    perm[i] = i;
+    pv[i] = p.piv[i];
+  }
   __syncthreads();
   if (t == 0)
     for (int k = 0; k < n; k++) {
-      const int q = p.piv[k];
+      const int q = pv[k];
       if (q != k) {
         const int x = perm[k];
         perm[k] = perm[q];
