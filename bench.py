@@ -292,9 +292,10 @@ def main():
         "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": f"gauss2d_kd N={n} gaussian l=0.05 shift=1e-4 leaf=256 "
                                f"aca_tol=1e-10 nrhs={nrhs} target={a.target:g}",
-                   "path": "learned rank sizing + FP64 recSM + FGMRES refinement with "
-                           "double-double residuals (the FP32 factorization does not "
-                           "converge at this condition number: see mixed)"},
+                   "path": "learned rank sizing + FP64 recSM + refinement x += M^-1 r with "
+                           "double-double residuals (flexible GMRES when a step gains less "
+                           "than 8x; the FP32 factorization does not converge at this "
+                           "condition number: see mixed)"},
         "phases_s": {"compress": st["compress_time"], "factor": st["factor_time"],
                      "solve_and_refine": st["solve_time"]},
         "refine_steps": st["refine_iterations"],

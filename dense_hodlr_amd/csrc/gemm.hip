@@ -186,8 +186,8 @@ __global__ void __launch_bounds__(256)
   constexpr int BM = 128, BN = 64, NT = 256;
   constexpr int A_PER = BM * BK / NT;
   constexpr int B_PER = BN * BK / NT;
-  __shared__ double As[2][BK][BM + 2];
-  __shared__ double Bs[2][BK][BN + 2];
+  __shared__ double As[BK][BM + 2];
+  __shared__ double Bs[BK][BN + 2];
   const int gtile = xcd_tile();
   const GemmDescT<double> g = descs[find_problem(descs, count, gtile)];
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
@@ -244,7 +244,7 @@ __global__ void __launch_bounds__(256)
       rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : 0.0;
     }
   };
-  auto store = [&](int buf) {
+  auto store = [&]() {
 #pragma unroll
     for (int r = 0; r < A_PER; r++) {
       const int e = t + r * NT;
@@ -256,40 +256,37 @@ __global__ void __launch_bounds__(256)
         mm = e % BM;
         kk = e / BM;
       }
-      As[buf][kk][mm] = ra[r];
+      As[kk][mm] = ra[r];
     }
 #pragma unroll
     for (int r = 0; r < B_PER; r++) {
       const int e = t + r * NT;
-      Bs[buf][e % BK][e / BK] = rb[r];
+      Bs[e % BK][e / BK] = rb[r];
     }
   };
 
-  int buf = 0;
-  if (kbeg < kend) {
-    load(kbeg);
-    store(0);
-  }
-  __syncthreads();
+  // one LDS slice; the next slice's loads are in flight in registers while
+  // the current one is multiplied (half the LDS of double buffering, so more
+  // workgroups fit per CU)
+  if (kbeg < kend) load(kbeg);
   for (int k0 = kbeg; k0 < kend; k0 += BK) {
-    const bool more = k0 + BK < kend;
-    if (more) load(k0 + BK);
+    __syncthreads();
+    store();
+    __syncthreads();
+    if (k0 + BK < kend) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
       double av[4], bv[2];
 #pragma unroll
-      for (int a = 0; a < 4; a++) av[a] = As[buf][kk + lk][wm + a * 16 + lr];
+      for (int a = 0; a < 4; a++) av[a] = As[kk + lk][wm + a * 16 + lr];
 #pragma unroll
-      for (int b = 0; b < 2; b++) bv[b] = Bs[buf][kk + lk][wn + b * 16 + lr];
+      for (int b = 0; b < 2; b++) bv[b] = Bs[kk + lk][wn + b * 16 + lr];
 #pragma unroll
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
           acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
     }
-    if (more) store(buf ^ 1);
-    __syncthreads();
-    buf ^= 1;
   }
 #pragma unroll
   for (int a = 0; a < 4; a++)
