@@ -68,6 +68,10 @@ std::map<std::string, Tuning> g_tune;  // autotuned shapes by problem shape
 
 Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   trace_ = std::getenv("HODLR_TRACE") != nullptr;
+  // largest system the one-launch LU solve takes (lowered to test the
+  // blocked path on small problems)
+  if (const char* e = std::getenv("HODLR_LU_SOLVE_SMALL_MAX"))
+    solve_small_max_ = std::max(0, std::min(LU_SOLVE_SMALL_MAX, std::atoi(e)));
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreate(&ev0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
@@ -455,7 +459,7 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
   if (descs.empty()) return;
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
-  if (maxn <= LU_SOLVE_SMALL_MAX) {  // every system in one launch
+  if (maxn <= solve_small_max_) {  // every system in one launch
     int nwg = 0;
     for (auto& d : descs) {
       d.chunk0 = nwg;

@@ -291,6 +291,7 @@ class Engine {
   double pred_margin_ = 0.25;
   std::vector<double> pred_rank_;
   bool trace_ = false;
+  int solve_small_max_ = LU_SOLVE_SMALL_MAX;  // one-launch LU solves up to this size
   std::vector<hipEvent_t> prof_ev_;
 
   hipStream_t stream_ = nullptr;

@@ -305,6 +305,27 @@ def test_launch_shapes_match_oracle(oracle, shapes):
     assert g.stats()["tuning"]["gemm_mfma"] == bool(shapes.get("gemm_mfma", False))
 
 
+@pytest.mark.parametrize("limit", [0, 40])
+def test_blocked_lu_solves_match_oracle(oracle, monkeypatch, limit):
+    """Systems above the one-launch solve's size limit (2048 rows; the root
+    capacitance of large problems) take the blocked path: permutation, panel
+    TRSM and GEMM updates.  Lowering the limit for a handle (0: every system,
+    40: leaves and deep nodes one-launch, the rest blocked) routes the
+    workload through that path; ranks, solutions and log-det stay the
+    oracle's."""
+    _case(f"blocked LU solves below limit {limit}")
+    monkeypatch.setenv("HODLR_LU_SOLVE_SMALL_MAX", str(limit))
+    cfg = I.config_inputs("gauss1d_4096")
+    g, o = _pair(oracle, cfg["n"], cfg["points"], "gaussian", 0.1, 1e-2, 64, 1e-10)
+    b = I.rhs_uniform(cfg["n"], 3, 7)
+    xs = g.recSM_Solve(b)
+    xl = g.recLU_Solve(b)
+    xo = o.recSM_Solve(b)
+    assert np.array_equal(g.nodes(), o.nodes())
+    assert rel(xs, xo) < 1e-10 and rel(xl, xo) < 1e-9
+    assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-11 * abs(o.logAbsDeterminant())
+
+
 def test_matern2d_65536_against_oracle_summary():
     """The N=65536 Matern-3/2 workload at full size (16 right-hand sides, the
     log-determinant) against the oracle's recSM on the same seeded inputs,
