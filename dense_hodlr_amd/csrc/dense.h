@@ -162,6 +162,8 @@ void launch_multi_dot(const double* V, size_t vstride, int nvec, const double* w
 void launch_multi_axpy(const double* V, size_t vstride, int nvec, const double* coef, double* w,
                        int n, int ld, int ncols, double sign, int assign, hipStream_t s);
 void launch_col_scale(double* X, int ld, int n, int ncols, const double* sc, hipStream_t s);
+/* x <- fl(x + d); d <- (x_new - x_old) up to one rounding (n x ncols, ld) */
+void launch_step_update(double* x, double* d, int ld, int n, int ncols, hipStream_t s);
 /* Adds the number of non-finite entries of x[0, n) to *out (diagnostics). */
 template <class T>
 void launch_nonfinite(const T* x, size_t n, unsigned long long* out, hipStream_t s);

@@ -553,6 +553,24 @@ __global__ void __launch_bounds__(TPB)
   }
 }
 
+/* x <- fl(x + d) and d <- the exact step x_new - x_old rounded once:
+ * TwoSum gives x_old + d = x_new + e exactly, so x_new - x_old = d - e. */
+__global__ void __launch_bounds__(TPB) step_update_kernel(double* __restrict__ x,
+                                                          double* __restrict__ d, int ld, int n,
+                                                          int ncols) {
+#pragma clang fp contract(off)
+  const size_t tot = (size_t)n * ncols;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < tot; e += (size_t)gridDim.x * TPB) {
+    const size_t o = (size_t)(e % n) + (size_t)(e / n) * ld;
+    const double a = x[o], b = d[o];
+    const double s = a + b;
+    const double bb = s - a;
+    const double err = (a - (s - bb)) + (b - bb);
+    x[o] = s;
+    d[o] = b - err;
+  }
+}
+
 __global__ void __launch_bounds__(TPB) col_scale_kernel(double* __restrict__ X, int ld, int n,
                                                         int ncols, const double* __restrict__ s) {
   const size_t tot = (size_t)n * ncols;
@@ -884,6 +902,12 @@ void launch_multi_dot(const double* V, size_t vstride, int nvec, const double* w
 void launch_multi_axpy(const double* V, size_t vstride, int nvec, const double* coef, double* w,
                        int n, int ld, int ncols, double sign, int assign, hipStream_t s) {
   multi_axpy_kernel<<<2048, TPB, 0, s>>>(V, vstride, nvec, coef, w, n, ld, ncols, sign, assign);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_step_update(double* x, double* d, int ld, int n, int ncols, hipStream_t s) {
+  if ((size_t)n * ncols == 0) return;
+  step_update_kernel<<<2048, TPB, 0, s>>>(x, d, ld, n, ncols);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

@@ -1843,15 +1843,37 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     double prev = 0.0;
     segnorms(rbuf_.p, rs);
     for (size_t e = 0; e < rs.size(); e++) prev = std::max(prev, rs[e] / bn[e]);
+    std::vector<double> dn, xn;
     while (prev > refine_target_ && total < refine_max_) {
-      precond(rbuf_.p, zbasis_.p);
-      launch_multi_axpy(zbasis_.p, vstride, 1, dcoef, xw_.p, ntot_, ld, nrhs, 1.0, 0, stream_);
-      residual_dd(Beff, ldbeff, xw_.p, ld, nrhs, rbuf_.p, ld);
+      precond(rbuf_.p, zbasis_.p);  // the step d = M^-1 r
+      // A step small against the iterate (every instance and column) updates
+      // the residual exactly enough in FP64: r_new = r - A_H (x_new - x_old),
+      // with x_new - x_old = d - e from the TwoSum x_old + d = x_new + e.  The
+      // FP64 product of the step errs by its size relative to x times the
+      // FP64 evaluation floor of A_H x (~1e-10 at N = 262144), i.e. by
+      // 1e-3 of that floor at most; r itself came exact from double-double
+      // arithmetic (the first step, from x = 0, always takes that path).
+      segnorms(zbasis_.p, dn);
+      segnorms(xw_.p, xn);
+      double ratio = 0.0;
+      for (size_t e = 0; e < dn.size(); e++)
+        ratio = std::max(ratio, xn[e] > 0 ? dn[e] / xn[e] : (dn[e] > 0 ? HUGE_VAL : 0.0));
+      const bool update = ratio <= 1e-3 && refine_target_ >= 1e-12;
+      if (update) {
+        launch_step_update(xw_.p, zbasis_.p, ld, ntot_, nrhs, stream_);
+        matvec_device(zbasis_.p, ld, nrhs, V, ld);
+        launch_multi_axpy(V, vstride, 1, dcoef, rbuf_.p, ntot_, ld, nrhs, -1.0, 0, stream_);
+      } else {
+        launch_multi_axpy(zbasis_.p, vstride, 1, dcoef, xw_.p, ntot_, ld, nrhs, 1.0, 0, stream_);
+        residual_dd(Beff, ldbeff, xw_.p, ld, nrhs, rbuf_.p, ld);
+      }
       total++;
       segnorms(rbuf_.p, rs);
       double now = 0.0;
       for (size_t e = 0; e < rs.size(); e++) now = std::max(now, rs[e] / bn[e]);
-      if (trace_) std::fprintf(stderr, "[refine] step %d true residual %.3e\n", total, now);
+      if (trace_)
+        std::fprintf(stderr, "[refine] step %d true residual %.3e (%s, step/iterate %.1e)\n", total,
+                     now, update ? "FP64 update" : "double-double", ratio);
       if (!std::isfinite(now)) throw std::runtime_error("refinement produced a non-finite residual");
       const bool gained = now <= prev / 8;
       prev = now;

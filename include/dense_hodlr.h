@@ -157,9 +157,13 @@ int hodlr_set_min_pivot(hodlr_handle h, double min_pivot);
 /* Factorization arithmetic (HODLR_PREC_*).  No reference counterpart. */
 int hodlr_set_precision(hodlr_handle h, int precision);
 /* Refinement (MIXED, FP64_REFINED): relative residual ||b - A_H x|| / ||b||
- * every column must reach (default 1e-10; the residual is evaluated in
- * double-double arithmetic, as hodlr_residual does) and the maximum number of
- * flexible GMRES steps (default 30; restart length min(20, max_iters)). */
+ * every column must reach (default 1e-10) and the maximum number of steps
+ * (default 30).  Residuals are exact to rounding: evaluated in double-double
+ * arithmetic, as hodlr_residual does, or, after a step at most 1e-3 of the
+ * iterate (targets >= 1e-12), updated by the FP64 product of the exact step
+ * (TwoSum).  FP64_REFINED takes steps x += M^-1 r while each gains 8x and
+ * then flexible GMRES (restart length min(20, max_iters)); MIXED uses
+ * flexible GMRES throughout. */
 int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
 
 /* Autotuner: times candidate launch shapes on this handle's own problem

@@ -64,7 +64,11 @@ def test_refined_gauss2d_reaches_1e10(prec, budget):
           f"{np.linalg.norm(xm - x64) / np.linalg.norm(x64):.2e}", flush=True)
     assert r64 > 1e-9  # the unrefined solve does not reach the target here
     assert st["last_residual"] <= 1e-10, st
-    assert colres(g, xm, b) <= 1.5e-10
+    rind = colres(g, xm, b)
+    assert rind <= 1.5e-10
+    # the residual the loop reports (updated in FP64 after small steps) is
+    # the true residual of the returned iterate
+    assert abs(st["last_residual"] - rind) <= 1e-13, (st["last_residual"], rind)
     assert st["refine_iterations"] <= budget
 
 
