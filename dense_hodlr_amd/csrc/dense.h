@@ -125,10 +125,11 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 /* Whole solves of systems of at most LU_SOLVE_SMALL_MAX rows in one launch
  * (interchanges, both substitutions); SolveDesc::chunk0 numbers the
- * workgroups, lu_solve_small_wgs(ncols) per problem.  Returns false, and
+ * workgroups, lu_solve_small_wgs(ncols, maxn) per problem (maxn: the largest
+ * system of the launch).  Returns false, and
  * launches nothing, when maxn exceeds the limit. */
 constexpr int LU_SOLVE_SMALL_MAX = 2048;
-int lu_solve_small_wgs(int ncols);
+int lu_solve_small_wgs(int ncols, int maxn);
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s);
@@ -145,11 +146,12 @@ void launch_balance(const BalanceDesc* d, int count, int kmax, hipStream_t s);
 /* out[c] = sum_i X[i + c*ld]^2 for c < ncols (one workgroup per column, fixed
  * reduction order) */
 void launch_colnorm2(const double* X, int n, int ld, int ncols, double* out, hipStream_t s);
-/* Per instance segment: out[s + c*nseg] = sum of squares of rows
- * [row0[s], row0[s+1]) of column c; segscale multiplies those rows by
+/* Per instance segment: out[p + nsplit*(s + c*nseg)] = sum of squares of
+ * the p-th of nsplit equal parts of rows [row0[s], row0[s+1]) of column c
+ * (the caller adds the parts in order); segscale multiplies those rows by
  * sc[s + c*nseg].  row0 has nseg + 1 entries (device). */
-void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, double* out,
-                     hipStream_t s);
+void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, int nsplit,
+                     double* out, hipStream_t s);
 void launch_segscale(double* X, int ld, const int* row0, int nseg, int ncols, const double* sc,
                      hipStream_t s);
 /* X += alpha * D (FP32 correction) over an n x ncols block */

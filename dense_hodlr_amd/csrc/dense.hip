@@ -461,17 +461,30 @@ __global__ void __launch_bounds__(TPB)
     segnorm2_kernel(const double* __restrict__ X, int ld, const int* __restrict__ row0, int nseg,
                     double* __restrict__ out) {
   __shared__ double sh[TPB];
-  const int sg = blockIdx.y, c = blockIdx.x;
+  const int sg = blockIdx.y, c = blockIdx.x, sp = blockIdx.z, nsp = gridDim.z;
   const double* x = X + (size_t)c * ld;
-  double acc = 0.0;
-  for (int i = row0[sg] + threadIdx.x; i < row0[sg + 1]; i += TPB) acc += x[i] * x[i];
+  // split sp of nsp of the segment's rows; four loads in flight per thread
+  const int r0 = row0[sg], len = row0[sg + 1] - r0;
+  const int part = (len + nsp - 1) / nsp;
+  const int lo = r0 + min(len, sp * part), hi = r0 + min(len, (sp + 1) * part);
+  double a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
+  int i = lo + threadIdx.x;
+  for (; i + 3 * TPB < hi; i += 4 * TPB) {
+    const double v0 = x[i], v1 = x[i + TPB], v2 = x[i + 2 * TPB], v3 = x[i + 3 * TPB];
+    a0 += v0 * v0;
+    a1 += v1 * v1;
+    a2 += v2 * v2;
+    a3 += v3 * v3;
+  }
+  for (; i < hi; i += TPB) a0 += x[i] * x[i];
+  const double acc = (a0 + a1) + (a2 + a3);
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = TPB / 2; w > 0; w >>= 1) {
     if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
-  if (threadIdx.x == 0) out[sg + (size_t)c * nseg] = sh[0];
+  if (threadIdx.x == 0) out[sp + (size_t)nsp * (sg + (size_t)c * nseg)] = sh[0];
 }
 
 __global__ void __launch_bounds__(TPB)
@@ -599,20 +612,27 @@ int grid_x(size_t elems, int ndesc) {
 
 /* grid.y is limited to 65535: descriptors are launched in chunks */
 /* Whole LU solve A X = B (row interchanges, unit lower, upper) of small
- * systems in one launch: one workgroup per problem and SOLVE_CB columns of B.
+ * systems in one launch: one workgroup per problem and CB columns of B.
  * Thread t keeps rows t + 256 r (r < R) of its columns in registers for the
  * whole solve; each substitution step the owner of the pivot row publishes
  * x_l through LDS (double-buffered by step parity, one barrier per step) and
- * every thread updates its rows with column l of the factor, whose entries
- * for the next step are loaded during the current one. */
-constexpr int SOLVE_CB = 8;
+ * every thread updates its rows with column l of the factor.  The factor's
+ * columns are loaded D steps ahead into a register ring (the step loop is
+ * unrolled by D, so the ring is indexed statically): a step then waits for
+ * a load issued D steps earlier, not for one issued in the step before.
+ * Fewer columns per workgroup for larger systems (more workgroups for the
+ * few large capacitance systems of the upper tree levels). */
+constexpr int solve_rows(int maxn) { return maxn <= 256 ? 1 : maxn <= 512 ? 2 : maxn <= 1024 ? 4 : 8; }
+constexpr int solve_cols(int R) { return R == 1 ? 8 : R == 8 ? 2 : 4; }
+constexpr int SOLVE_D = 8;
 
 template <int R>
 __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __restrict__ d,
                                                              int count) {
+  constexpr int CB = solve_cols(R), D = SOLVE_D;
   __shared__ int perm[R * TPB];
   __shared__ int pv[R * TPB];
-  __shared__ double xs[2][SOLVE_CB];
+  __shared__ double xs[2][CB];
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
@@ -623,8 +643,8 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
   }
   const SolveDesc p = d[lo];
   const int n = p.n, t = threadIdx.x;
-  const int c0 = ((int)blockIdx.x - p.chunk0) * SOLVE_CB;
-  const int nc = min(SOLVE_CB, p.ncols - c0);
+  const int c0 = ((int)blockIdx.x - p.chunk0) * CB;
+  const int nc = min(CB, p.ncols - c0);
   // the interchanges piv[0..n), applied in order, as one permutation (read
   // by all threads, replayed from LDS by one)
   for (int i = t; i < n; i += TPB) {
@@ -642,81 +662,99 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
       }
     }
   __syncthreads();
-  double x[R][SOLVE_CB];
+  double x[R][CB];
 #pragma unroll
   for (int r = 0; r < R; r++) {
     const int i = t + TPB * r;
 #pragma unroll
-    for (int c = 0; c < SOLVE_CB; c++)
+    for (int c = 0; c < CB; c++)
       x[r][c] = (i < n && c < nc) ? p.B[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
   }
   const double* LU = p.LU;
   const size_t ld = (size_t)p.ld_lu;
-  // forward: unit lower triangle
-  double f[R];
+  double fr[D][R];
+  // forward: unit lower triangle; fr[s] holds column l (rows below the
+  // diagonal) for the steps l = s (mod D)
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int i = t + TPB * r;
-    f[r] = (i > 0 && i < n) ? LU[(size_t)i] : 0.0;  // column 0
-  }
-  for (int l = 0; l < n; l++) {
-    const int own = l / TPB;
-    if (t == l % TPB) {
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        if (r == own)
-#pragma unroll
-          for (int c = 0; c < SOLVE_CB; c++) xs[l & 1][c] = x[r][c];
-    }
-    double fn[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {  // column l + 1, rows below the diagonal
-      const int i = t + TPB * r;
-      fn[r] = (l + 1 < n && i > l + 1 && i < n) ? LU[(size_t)i + (size_t)(l + 1) * ld] : 0.0;
-    }
-    __syncthreads();
+  for (int s = 0; s < D; s++)
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = t + TPB * r;
-      if (i > l && i < n)
+      fr[s][r] = (s < n && i > s && i < n) ? LU[(size_t)i + (size_t)s * ld] : 0.0;
+    }
+  for (int l0 = 0; l0 < n; l0 += D) {
 #pragma unroll
-        for (int c = 0; c < SOLVE_CB; c++) x[r][c] -= f[r] * xs[l & 1][c];
-      f[r] = fn[r];
+    for (int s = 0; s < D; s++) {
+      const int l = l0 + s;
+      if (l < n) {
+        const int own = l / TPB;
+        if (t == l % TPB) {
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            if (r == own)
+#pragma unroll
+              for (int c = 0; c < CB; c++) xs[l & 1][c] = x[r][c];
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = t + TPB * r;
+          if (i > l && i < n)
+#pragma unroll
+            for (int c = 0; c < CB; c++) x[r][c] -= fr[s][r] * xs[l & 1][c];
+        }
+        const int ln = l + D;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = t + TPB * r;
+          fr[s][r] = (ln < n && i > ln && i < n) ? LU[(size_t)i + (size_t)ln * ld] : 0.0;
+        }
+      }
     }
   }
   __syncthreads();
-  // backward: upper triangle with its diagonal
+  // backward: upper triangle with its diagonal; fr[s] holds column l (rows
+  // on and above the diagonal) for the steps l = n - 1 - s (mod D)
 #pragma unroll
-  for (int r = 0; r < R; r++) {
-    const int i = t + TPB * r;
-    f[r] = (n > 0 && i <= n - 1 && i < n) ? LU[(size_t)i + (size_t)(n - 1) * ld] : 0.0;
-  }
-  for (int l = n - 1; l >= 0; l--) {
-    const int own = l / TPB;
-    if (t == l % TPB) {
-#pragma unroll
-      for (int r = 0; r < R; r++)
-        if (r == own)
-#pragma unroll
-          for (int c = 0; c < SOLVE_CB; c++) {
-            x[r][c] = x[r][c] / f[r];
-            xs[l & 1][c] = x[r][c];
-          }
-    }
-    double fn[R];
-#pragma unroll
-    for (int r = 0; r < R; r++) {  // column l - 1, rows on and above the diagonal
-      const int i = t + TPB * r;
-      fn[r] = (l >= 1 && i <= l - 1) ? LU[(size_t)i + (size_t)(l - 1) * ld] : 0.0;
-    }
-    __syncthreads();
+  for (int s = 0; s < D; s++) {
+    const int l = n - 1 - s;
 #pragma unroll
     for (int r = 0; r < R; r++) {
       const int i = t + TPB * r;
-      if (i < l)
+      fr[s][r] = (l >= 0 && i <= l) ? LU[(size_t)i + (size_t)l * ld] : 0.0;
+    }
+  }
+  for (int l0 = n - 1; l0 >= 0; l0 -= D) {
 #pragma unroll
-        for (int c = 0; c < SOLVE_CB; c++) x[r][c] -= f[r] * xs[l & 1][c];
-      f[r] = fn[r];
+    for (int s = 0; s < D; s++) {
+      const int l = l0 - s;
+      if (l >= 0) {
+        const int own = l / TPB;
+        if (t == l % TPB) {
+#pragma unroll
+          for (int r = 0; r < R; r++)
+            if (r == own)
+#pragma unroll
+              for (int c = 0; c < CB; c++) {
+                x[r][c] = x[r][c] / fr[s][r];
+                xs[l & 1][c] = x[r][c];
+              }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = t + TPB * r;
+          if (i < l)
+#pragma unroll
+            for (int c = 0; c < CB; c++) x[r][c] -= fr[s][r] * xs[l & 1][c];
+        }
+        const int ln = l - D;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = t + TPB * r;
+          fr[s][r] = (ln >= 0 && i <= ln) ? LU[(size_t)i + (size_t)ln * ld] : 0.0;
+        }
+      }
     }
   }
 #pragma unroll
@@ -724,7 +762,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
     const int i = t + TPB * r;
     if (i < n)
 #pragma unroll
-      for (int c = 0; c < SOLVE_CB; c++)
+      for (int c = 0; c < CB; c++)
         if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
   }
 }
@@ -775,19 +813,20 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-int lu_solve_small_wgs(int ncols) { return (ncols + SOLVE_CB - 1) / SOLVE_CB; }
+int lu_solve_small_wgs(int ncols, int maxn) {
+  const int cb = solve_cols(solve_rows(maxn));
+  return (ncols + cb - 1) / cb;
+}
 
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s) {
   if (maxn > LU_SOLVE_SMALL_MAX) return false;
   if (count == 0 || nwg == 0) return true;
-  if (maxn <= TPB)
-    lu_solve_small_kernel<1><<<nwg, TPB, 0, s>>>(d, count);
-  else if (maxn <= 2 * TPB)
-    lu_solve_small_kernel<2><<<nwg, TPB, 0, s>>>(d, count);
-  else if (maxn <= 4 * TPB)
-    lu_solve_small_kernel<4><<<nwg, TPB, 0, s>>>(d, count);
-  else
-    lu_solve_small_kernel<8><<<nwg, TPB, 0, s>>>(d, count);
+  switch (solve_rows(maxn)) {
+    case 1: lu_solve_small_kernel<1><<<nwg, TPB, 0, s>>>(d, count); break;
+    case 2: lu_solve_small_kernel<2><<<nwg, TPB, 0, s>>>(d, count); break;
+    case 4: lu_solve_small_kernel<4><<<nwg, TPB, 0, s>>>(d, count); break;
+    default: lu_solve_small_kernel<8><<<nwg, TPB, 0, s>>>(d, count); break;
+  }
   HODLR_HIP_CHECK(hipGetLastError());
   return true;
 }
@@ -864,10 +903,11 @@ void launch_nonfinite(const T* x, size_t n, unsigned long long* out, hipStream_t
 template void launch_nonfinite<float>(const float*, size_t, unsigned long long*, hipStream_t);
 template void launch_nonfinite<double>(const double*, size_t, unsigned long long*, hipStream_t);
 
-void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, double* out,
-                     hipStream_t s) {
+void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int ncols, int nsplit,
+                     double* out, hipStream_t s) {
   if (ncols <= 0 || nseg <= 0) return;
-  dim3 g(ncols, nseg);
+  if (nsplit < 1 || nsplit > 64) throw std::invalid_argument("segnorm2: nsplit out of range");
+  dim3 g(ncols, nseg, nsplit);
   segnorm2_kernel<<<g, TPB, 0, s>>>(X, ld, row0, nseg, out);
   HODLR_HIP_CHECK(hipGetLastError());
 }

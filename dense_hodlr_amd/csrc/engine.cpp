@@ -463,7 +463,7 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
     int nwg = 0;
     for (auto& d : descs) {
       d.chunk0 = nwg;
-      nwg += lu_solve_small_wgs(d.ncols);
+      nwg += lu_solve_small_wgs(d.ncols, maxn);
     }
     launch_lu_solve_small(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, stream_);
     return;
@@ -1759,7 +1759,10 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     seg_rows_.alloc(r0.size());
     HODLR_HIP_CHECK(hipMemcpy(seg_rows_.p, r0.data(), r0.size() * sizeof(int), hipMemcpyHostToDevice));
   }
-  grow(segbuf_, (size_t)nseg * nrhs);
+  // norms: enough workgroups to stream the vectors at bandwidth (parts
+  // summed on the host in order)
+  const int nsplit = std::max(1, std::min(16, 2048 / std::max(1, nseg * nrhs)));
+  grow(segbuf_, (size_t)nseg * nrhs * nsplit);
   auto colnorms = [&](const double* X, std::vector<double>& out) {
     launch_colnorm2(X, ntot_, ld, nrhs, dnorm, stream_);
     out.resize(nrhs);
@@ -1767,12 +1770,18 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
     for (auto& v : out) v = std::sqrt(v);
   };
+  std::vector<double> parts;
   auto segnorms = [&](const double* X, std::vector<double>& out) {
-    launch_segnorm2(X, ld, seg_rows_.p, nseg, nrhs, segbuf_.p, stream_);
-    out.resize((size_t)nseg * nrhs);
-    HODLR_HIP_CHECK(hipMemcpyAsync(out.data(), segbuf_.p, out.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
+    launch_segnorm2(X, ld, seg_rows_.p, nseg, nrhs, nsplit, segbuf_.p, stream_);
+    parts.resize((size_t)nseg * nrhs * nsplit);
+    HODLR_HIP_CHECK(hipMemcpyAsync(parts.data(), segbuf_.p, parts.size() * sizeof(double), hipMemcpyDeviceToHost, stream_));
     HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
-    for (auto& v : out) v = std::sqrt(v);
+    out.assign((size_t)nseg * nrhs, 0.0);
+    for (size_t e = 0; e < out.size(); e++) {
+      double v = 0.0;
+      for (int p = 0; p < nsplit; p++) v += parts[e * nsplit + p];
+      out[e] = std::sqrt(v);
+    }
   };
   auto upload = [&](const std::vector<double>& h, double* d) {
     HODLR_HIP_CHECK(hipMemcpyAsync(d, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice, stream_));
