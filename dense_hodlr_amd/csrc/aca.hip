@@ -239,7 +239,8 @@ __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, A
   s.eps = 1.0;
   s.iters = 0;
   s.pending = 0;
-  s.pad_[0] = s.pad_[1] = 0;
+  s.fin_chunks = 1;
+  s.pad_ = 0;
   st[b] = s;
   bits[blocks[b].bits_row0] |= 1u;  // row 0 is chosen first (lowRank.cpp:159)
 }
@@ -322,7 +323,12 @@ __global__ void __launch_bounds__(TPB)
   const int k = s.k;
   const bool pend = s.pending != 0;
   const int j = decide_column(L, b, bits + b.bits_col0, min_pivot, pr_sh);
-  if (local == 0 && t == 0) st[bidx].cur_col = j;
+  if (local == 0 && t == 0) {
+    st[bidx].cur_col = j;
+    // the finalize's chunks of dot products l < k - 1, fixed here so that
+    // its workgroups agree whether or not its last one has updated k
+    st[bidx].fin_chunks = max(1, (k - 1 + FIN_L - 1) / FIN_L);
+  }
   double* __restrict__ Wp = L.W[b.level] + b.w_row0;
   const double* __restrict__ Vp = L.V[b.level] + b.v_row0;
   const int ibase = local * TPB * IPT + t;
@@ -401,6 +407,11 @@ __global__ void __launch_bounds__(TPB)
   const int k = s.k;
   const int j = s.cur_col;
   const bool pend = s.pending != 0;
+  // chunks that hold dot products this iteration (l < k - 1, fixed by the
+  // column phase); the others leave at once, so early iterations of blocks
+  // with large capacity do not pay for their empty chunks
+  const int nact = min(nchunk[bidx], s.fin_chunks);
+  if (chunk >= nact) return;
 
   // cross term of the pending term k-1 (lowRank.cpp:234-237): sum_{l<k-1} e_l d_l.
   // FIN_L values of l per workgroup, each summed over the slots by 4 groups.
@@ -428,20 +439,25 @@ __global__ void __launch_bounds__(TPB)
     }
   }
   cross = wg_sum(cross, red);
-  if (t == 0) {
-    fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = cross;
+  // With one chunk this workgroup is the block's only one: no partials to
+  // publish, no device-scope fences (each writes back this XCD's L2).
+  if (nact > 1) {
+    if (t == 0) {
+      fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = cross;
+      __threadfence();
+      const int old = atomicAdd(&counters[bidx], 1);
+      last = (old == nact - 1);
+    }
+    __syncthreads();
+    if (!last) return;
+    // the last workgroup of the block: all its threads reduce, thread 0 decides
     __threadfence();
-    const int old = atomicAdd(&counters[bidx], 1);
-    last = (old == nchunk[bidx] - 1);
   }
-  __syncthreads();
-  if (!last) return;
-  // the last workgroup of the block: all its threads reduce, thread 0 decides
-  __threadfence();
   const int cap = L.cap[b.level];
   double crossSum = 0.0, nv2 = 0.0, nw2 = 0.0;
   if (pend) {
-    for (int c = t; c < nchunk[bidx]; c += TPB) crossSum += fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
+    for (int c = t; c < nact; c += TPB)
+      crossSum += nact == 1 ? cross : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
     for (int w = t; w < b.row_nwg; w += TPB) nv2 += slot_ptr(L, b, w)[k - 1];
     for (int w = t; w < b.col_nwg; w += TPB) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
     crossSum = wg_sum(crossSum, red);

@@ -70,7 +70,8 @@ struct AcaState {
   double eps;   /* last ||w|| ||v|| / ||W V^T||_F */
   int iters;    /* iterations executed (including row jumps) */
   int pending;  /* 1: term k-1 is stored, its epsilon is completed next iteration */
-  int pad_[2];
+  int fin_chunks; /* finalize chunks with dot products this iteration (column phase) */
+  int pad_;
 };
 
 /* One problem of a grouped GEMM: C = alpha * op(A) * B + beta * C,
