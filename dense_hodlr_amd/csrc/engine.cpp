@@ -1126,10 +1126,23 @@ void Engine::factorize_fp64(int which) {
   factor_leaves();
   const size_t N = (size_t)ntot_;
   F.W.alloc(std::max<size_t>(1, N * ktot_));
+  // With every leaf on one level (the default trees of power-of-two sizes),
+  // the leaf step reads the U panels and writes W directly: every row of W
+  // is a leaf row there, and its columns of that level and deeper are zero
+  // padding in U as in W's other columns of those rows.  Otherwise W starts
+  // as a copy of the panels.
+  int leaf_levels = 0, leaf_level = -1;
   for (int l = 0; l < nlev_; l++)
-    if (K_[l] > 0)
-      HODLR_HIP_CHECK(hipMemcpyAsync(F.W.p + N * off_[l], U_[l].p, N * K_[l] * sizeof(double),
-                                     hipMemcpyDeviceToDevice, stream_));
+    if (!level_leaves_[l].empty()) {
+      leaf_levels++;
+      leaf_level = l;
+    }
+  const bool direct = leaf_levels == 1 && off_[leaf_level] == ktot_;
+  if (!direct)
+    for (int l = 0; l < nlev_; l++)
+      if (K_[l] > 0)
+        HODLR_HIP_CHECK(hipMemcpyAsync(F.W.p + N * off_[l], U_[l].p, N * K_[l] * sizeof(double),
+                                       hipMemcpyDeviceToDevice, stream_));
   const int nn = (int)nodes_.size();
   F.c_off.assign(nn, 0);
   F.p_off.assign(nn, 0);
@@ -1148,7 +1161,17 @@ void Engine::factorize_fp64(int which) {
   for (int d = nlev_ - 1; d >= 0; d--) {
     const int offd = off_[d];
     // leaves of level d: W[rows, 0:off_d] <- A_leaf^-1 W[rows, 0:off_d]
-    if (offd > 0 && !level_leaves_[d].empty()) {
+    if (offd > 0 && !level_leaves_[d].empty() && direct) {
+      std::vector<GemmDesc> g;
+      for (int li : level_leaves_[d]) {
+        const LeafH& lf = leaves_[li];
+        for (int l = 0; l < d; l++)
+          if (K_[l] > 0)
+            g.push_back(gemm_desc(lf.s, K_[l], lf.s, 0, Linv_.p + lf.mat_off, lf.s, U_[l].p + lf.row0,
+                                  ld, W + lf.row0 + N * off_[l], ld, 1.0, 0.0));
+      }
+      run_gemm(g);
+    } else if (offd > 0 && !level_leaves_[d].empty()) {
       size_t need = 0;
       for (int li : level_leaves_[d]) need += (size_t)leaves_[li].s * offd;
       grow(tmp_, need);
