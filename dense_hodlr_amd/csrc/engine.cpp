@@ -421,11 +421,22 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
   for (auto& d : descs) maxn = std::max(maxn, d.n);
   const LuDesc* dd = arena_.push(descs, stream_);
   const int nb = lu_panel_width(maxn, tune_.lu_nb);
+  // Every panel step's TRSM and trailing-update descriptors are planned on
+  // the host first and uploaded together: one copy per batch instead of two
+  // or three per panel (a 1090-row matrix has 137 panels of 8).
+  struct Step {
+    int tr0, ntr, nwg;            // TRSM descriptors [tr0, tr0 + ntr)
+    int g0, ng, ntiles, bm;       // GEMM descriptors [g0, g0 + ng)
+    int id0, nid;                 // split-K problem ids [id0, id0 + nid)
+  };
+  std::vector<Step> steps;
+  std::vector<TrsmDesc> tr_all;
+  std::vector<GemmDesc> g_all;
+  std::vector<int> id_all;
+  size_t ws = 0;
   for (int p0 = 0; p0 < maxn; p0 += nb) {
-    launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, stream_);
-    std::vector<TrsmDesc> tr;
+    Step st{(int)tr_all.size(), 0, 0, (int)g_all.size(), 0, 0, 0, (int)id_all.size(), 0};
     std::vector<GemmDesc> g;
-    int nwg = 0;
     for (auto& d : descs) {
       const int pw = std::min(nb, d.n - p0);
       const int rest = d.n - p0 - pw;
@@ -438,17 +449,37 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
       t.pw = pw;
       t.upper = 0;
       t.ncols = rest;
-      t.chunk0 = nwg;
-      nwg += (rest + 63) / 64;
-      tr.push_back(t);
+      t.chunk0 = st.nwg;
+      st.nwg += (rest + 63) / 64;
+      tr_all.push_back(t);
       g.push_back(gemm_desc(rest, rest, pw, 0, d.A + (p0 + pw) + (size_t)p0 * d.ld, d.ld,
                             d.A + p0 + (size_t)(p0 + pw) * d.ld, d.ld,
                             d.A + (p0 + pw) + (size_t)(p0 + pw) * d.ld, d.ld, -1.0, 1.0));
     }
-    if (!tr.empty()) {
-      launch_trsm(arena_.push(tr, stream_), (int)tr.size(), nwg, stream_);
-      run_gemm(g);
+    st.ntr = (int)tr_all.size() - st.tr0;
+    if (!g.empty()) {
+      std::vector<int> ids;
+      st.bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(g);
+      ws = std::max(ws, gemm_plan(g, ids, st.ntiles, tune_.gemm_target, st.bm));
+      st.ng = (int)g.size();
+      st.nid = (int)ids.size();
+      g_all.insert(g_all.end(), g.begin(), g.end());
+      id_all.insert(id_all.end(), ids.begin(), ids.end());
     }
+    steps.push_back(st);
+  }
+  if (ws) ensure_ws(ws);
+  const TrsmDesc* dtr = tr_all.empty() ? nullptr : arena_.push(tr_all, stream_);
+  const GemmDesc* dg = g_all.empty() ? nullptr : arena_.push(g_all, stream_);
+  const int* did = id_all.empty() ? nullptr : arena_.push(id_all, stream_);
+  int p = 0;
+  for (int p0 = 0; p0 < maxn; p0 += nb, p++) {
+    launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, stream_);
+    const Step& st = steps[p];
+    if (st.ntr == 0) continue;
+    launch_trsm(dtr + st.tr0, st.ntr, st.nwg, stream_);
+    launch_gemm<double>(dg + st.g0, st.ng, st.ntiles, ws_.p, did ? did + st.id0 : nullptr, st.nid,
+                        st.bm, stream_, tune_.gemm_mfma != 0);
   }
 }
 
