@@ -206,11 +206,14 @@ template <int NT, int RPT, int PW>
 __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restrict__ d, int p0,
                                                           int nb) {
   constexpr int NW = NT / 64;
-  __shared__ double sv[NW];
-  __shared__ int si[NW];
+  // two barriers per column: LDS buffers alternate by column parity, every
+  // thread finishes the pivot search from the waves' partials itself, and
+  // the pivot row and the row it replaces are published together
+  __shared__ double sv[2][NW];
+  __shared__ int si[2][NW];
+  __shared__ double prow[2][PW];  // old row p: the pivot row after the exchange
+  __shared__ double crow[2][PW];  // old row cr: moves to row p
   __shared__ int piv_sh[PW];
-  __shared__ double xrow[2][PW];
-  __shared__ double prow[PW];
   const LuDesc m = d[blockIdx.x];
   const int n = m.n, ld = m.ld, t = threadIdx.x;
   if (n <= p0) return;
@@ -227,7 +230,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
 #pragma unroll
   for (int c = 0; c < PW; c++) {
     if (c < pw) {  // (uniform; a guard rather than a break keeps the loop unrolled)
-    const int cr = p0 + c;
+    const int cr = p0 + c, buf = c & 1;
     // first maximum of |A(i, cr)|, i >= cr
     double bv = -1.0;
     int bi = n;
@@ -251,53 +254,48 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
       }
     }
     if ((t & 63) == 0) {
-      sv[t >> 6] = bv;
-      si[t >> 6] = bi;
+      sv[buf][t >> 6] = bv;
+      si[buf][t >> 6] = bi;
     }
     __syncthreads();
+    double v = sv[buf][0];
+    int p = si[buf][0];
+    for (int w = 1; w < NW; w++)
+      if (sv[buf][w] > v || (sv[buf][w] == v && si[buf][w] < p)) {
+        v = sv[buf][w];
+        p = si[buf][w];
+      }
+    if (p >= n) p = cr;  // a column of NaNs has no maximum: keep the row
     if (t == 0) {
-      double v = sv[0];
-      int ix = si[0];
-      for (int w = 1; w < NW; w++)
-        if (sv[w] > v || (sv[w] == v && si[w] < ix)) {
-          v = sv[w];
-          ix = si[w];
-        }
-      if (ix >= n) ix = cr;  // a column of NaNs has no maximum: keep the row
-      piv_sh[c] = ix;
-      m.piv[cr] = ix;
+      m.piv[cr] = p;
+      piv_sh[c] = p;  // read after the loop, behind this column's second barrier
+    }
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      const int r = p0 + t + NT * q;
+      if (r == p)
+#pragma unroll
+        for (int j = 0; j < PW; j++) prow[buf][j] = R[q][j];
+      if (r == cr)
+#pragma unroll
+        for (int j = 0; j < PW; j++) crow[buf][j] = R[q][j];
     }
     __syncthreads();
-    const int p = piv_sh[c];
     // exchange rows cr and p of the panel
     if (p != cr) {
 #pragma unroll
       for (int q = 0; q < RPT; q++) {
         const int r = p0 + t + NT * q;
-        if (r == cr || r == p) {
+        if (r == cr) {
 #pragma unroll
-          for (int j = 0; j < PW; j++) xrow[r == cr ? 0 : 1][j] = R[q][j];
-        }
-      }
-      __syncthreads();
+          for (int j = 0; j < PW; j++) R[q][j] = prow[buf][j];
+        } else if (r == p) {
 #pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        const int r = p0 + t + NT * q;
-        if (r == cr || r == p) {
-#pragma unroll
-          for (int j = 0; j < PW; j++) R[q][j] = xrow[r == cr ? 1 : 0][j];
+          for (int j = 0; j < PW; j++) R[q][j] = crow[buf][j];
         }
       }
     }
-    // broadcast the pivot row
-#pragma unroll
-    for (int q = 0; q < RPT; q++)
-      if (p0 + t + NT * q == cr) {
-#pragma unroll
-        for (int j = 0; j < PW; j++) prow[j] = R[q][j];
-      }
-    __syncthreads();
-    const double pv = prow[c];
+    const double pv = prow[buf][c];
     if (pv != 0.0) {
       const double inv = 1.0 / pv;
 #pragma unroll
@@ -307,7 +305,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
           R[q][c] *= inv;
 #pragma unroll
           for (int j = 0; j < PW; j++)
-            if (j > c) R[q][j] -= R[q][c] * prow[j];
+            if (j > c) R[q][j] -= R[q][c] * prow[buf][j];
         }
       }
     } else {
@@ -317,11 +315,10 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
         if (r > cr && r < n) {
 #pragma unroll
           for (int j = 0; j < PW; j++)
-            if (j > c) R[q][j] -= R[q][c] * prow[j];
+            if (j > c) R[q][j] -= R[q][c] * prow[buf][j];
         }
       }
     }
-    __syncthreads();
     }
   }
 #pragma unroll
