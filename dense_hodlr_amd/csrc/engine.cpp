@@ -380,13 +380,14 @@ void Engine::run_gemm_t(std::vector<GemmDescT<T>>& descs) {
   if (descs.empty()) return;
   std::vector<int> split_ids;
   int ntiles = 0;
-  const int bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(descs);
-  const size_t ws = gemm_plan(descs, split_ids, ntiles, tune_.gemm_target, bm);
+  const bool mfma = tune_.gemm_mfma != 0;
+  const int bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(descs, mfma);
+  const size_t ws = gemm_plan(descs, split_ids, ntiles, tune_.gemm_target, bm, mfma);
   if (ws) ensure_ws((ws * sizeof(T) + 7) / 8);
   const GemmDescT<T>* d = arena_.push(descs, stream_);
   const int* ids = split_ids.empty() ? nullptr : arena_.push(split_ids, stream_);
   launch_gemm<T>(d, (int)descs.size(), ntiles, reinterpret_cast<T*>(ws_.p), ids,
-                 (int)split_ids.size(), bm, stream_, tune_.gemm_mfma != 0);
+                 (int)split_ids.size(), bm, stream_, mfma);
 }
 
 void Engine::run_gemm(std::vector<GemmDesc>& descs) { run_gemm_t<double>(descs); }
@@ -459,8 +460,9 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
     st.ntr = (int)tr_all.size() - st.tr0;
     if (!g.empty()) {
       std::vector<int> ids;
-      st.bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(g);
-      ws = std::max(ws, gemm_plan(g, ids, st.ntiles, tune_.gemm_target, st.bm));
+      const bool mfma = tune_.gemm_mfma != 0;
+      st.bm = tune_.gemm_bm ? tune_.gemm_bm : gemm_tile_m(g, mfma);
+      ws = std::max(ws, gemm_plan(g, ids, st.ntiles, tune_.gemm_target, st.bm, mfma));
       st.ng = (int)g.size();
       st.nid = (int)ids.size();
       g_all.insert(g_all.end(), g.begin(), g.end());

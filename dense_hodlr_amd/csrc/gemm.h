@@ -10,17 +10,24 @@
 
 namespace hodlr {
 
-/* Tile height (128 or 64) that suits a group of products. */
+/* Tile height (128 or 64) that suits a group of products.  The tiles are
+ * 128 x 64 or 64 x 64 on the vector units; FP64 on the matrix cores
+ * (mfma) uses 128 x 64 or 64 x 128, whichever pads the group less. */
 template <class T>
-int gemm_tile_m(const std::vector<GemmDescT<T>>& descs);
+int gemm_tile_m(const std::vector<GemmDescT<T>>& descs, bool mfma = false);
 
-/* Assigns tile offsets and K splits for tiles bm x 64; returns the workspace
- * size in elements. */
+/* Tile width that goes with tile height bm. */
+template <class T>
+int gemm_tile_n(int bm, bool mfma);
+
+/* Assigns tile offsets and K splits for tiles bm x gemm_tile_n(bm, mfma);
+ * returns the workspace size in elements. */
 template <class T>
 size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, int& ntiles,
-                 int target_tiles, int bm);
+                 int target_tiles, int bm, bool mfma = false);
 
-/* mfma (FP64, bm 128 only): run the tiles on the matrix cores */
+/* mfma (FP64): run the tiles on the matrix cores; bm and mfma as given to
+ * gemm_plan */
 template <class T>
 void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma = false);

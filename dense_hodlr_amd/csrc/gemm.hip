@@ -172,18 +172,26 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   }
 }
 
-/* FP64 on the matrix cores: the same tiles, descriptors and split-K contract
- * as gemm_kernel<double, 128, 64>, each wave computing a 64 x 32 sub-tile as
+/* FP64 on the matrix cores: the descriptors and split-K contract of
+ * gemm_kernel, tiles BM x BN of 128 x 64 or 64 x 128 (the latter for
+ * products with few rows: the V^T W products of the factorization, whose
+ * row count is a rank), each wave computing a 64 x 32 sub-tile as
  * 4 x 2 blocks of v_mfma_f64_16x16x4_f64.  Operand layout of that
  * instruction (wave of 64): lane l holds A[l % 16][l / 16] of the 16 x 4
  * A block and B[l / 16][l % 16] of the 4 x 16 B block; the 16 x 16 result
- * is held as D[l / 16 + 4 i][l % 16] in register i = 0..3. */
+ * is held as D[l / 16 + 4 i][l % 16] in register i = 0..3.  The blocks are
+ * multiplied transposed, D = B^T A^T = (A B)^T, so that lane l holds
+ * (A B)[l % 16][l / 16 + 4 i]: the 16 lanes of a quarter wave then read and
+ * write 16 consecutive rows of one column of the column-major C (a 128-byte
+ * line) instead of one element in each of 16 columns. */
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-__global__ void __launch_bounds__(256)
+template <int BM, int BN>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
     gemm_mfma_f64_kernel(const GemmDescT<double>* __restrict__ descs, int count,
                          double* __restrict__ ws) {
-  constexpr int BM = 128, BN = 64, NT = 256;
+  constexpr int NT = 256, WM = BM / 64;  // waves along M
+  static_assert(WM * (BN / 32) == 4, "four waves of 64 x 32");
   constexpr int A_PER = BM * BK / NT;
   constexpr int B_PER = BN * BK / NT;
   __shared__ double As[BK][BM + 2];
@@ -198,7 +206,7 @@ __global__ void __launch_bounds__(256)
   const int kchunk = ((g.K + g.ksplit - 1) / g.ksplit + BK - 1) / BK * BK;
   const int kbeg = split * kchunk, kend = min(g.K, kbeg + kchunk);
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  const int wm = (wave & 1) * 64, wn = (wave >> 1) * 32;
+  const int wm = (wave % WM) * 64, wn = (wave / WM) * 32;
   const int lr = lane & 15, lk = lane >> 4;
 
   // C += A B and C -= A B (the factorization's updates) start the sums from
@@ -212,7 +220,7 @@ __global__ void __launch_bounds__(256)
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int gm = m0 + wm + a * 16 + lk + 4 * i, gn = n0 + wn + b * 16 + lr;
+        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + lk + 4 * i;
         acc[a][b][i] = (cinit && gm < g.M && gn < g.N)
                            ? g.alpha * g.C[(size_t)gm + (size_t)gn * g.ldc]
                            : 0.0;
@@ -285,7 +293,7 @@ __global__ void __launch_bounds__(256)
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[a], bv[b], acc[a][b], 0, 0, 0);
+          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], acc[a][b], 0, 0, 0);
     }
   }
 #pragma unroll
@@ -294,7 +302,7 @@ __global__ void __launch_bounds__(256)
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int gm = m0 + wm + a * 16 + lk + 4 * i, gn = n0 + wn + b * 16 + lr;
+        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + lk + 4 * i;
         if (gm < g.M && gn < g.N) {
           const double v = acc[a][b][i];
           if (cinit) {
@@ -327,7 +335,16 @@ __global__ void __launch_bounds__(256)
 }  // namespace
 
 template <class T>
-int gemm_tile_m(const std::vector<GemmDescT<T>>& descs) {
+int gemm_tile_m(const std::vector<GemmDescT<T>>& descs, bool mfma) {
+  if (std::is_same<T, double>::value && mfma) {
+    // the same tile area either way: the shape that pads less
+    double a128 = 0, a64 = 0;
+    for (auto& g : descs) {
+      a128 += (double)((g.M + 127) / 128) * ((g.N + 63) / 64) * g.K;
+      a64 += (double)((g.M + 63) / 64) * ((g.N + 127) / 128) * g.K;
+    }
+    return a64 < a128 ? 64 : 128;
+  }
   // the wide tile when most of the work has more than 64 rows
   double big = 0, small = 0;
   for (auto& g : descs) {
@@ -341,9 +358,14 @@ int gemm_tile_m(const std::vector<GemmDescT<T>>& descs) {
 }
 
 template <class T>
+int gemm_tile_n(int bm, bool mfma) {
+  return (std::is_same<T, double>::value && mfma && bm == 64) ? 128 : 64;
+}
+
+template <class T>
 size_t gemm_plan(std::vector<GemmDescT<T>>& descs, std::vector<int>& split_ids, int& ntiles,
-                 int target_tiles, int bm) {
-  const int BM = bm, BN = 64;
+                 int target_tiles, int bm, bool mfma) {
+  const int BM = bm, BN = gemm_tile_n<T>(bm, mfma);
   size_t tiles_out = 0;
   for (auto& g : descs) {
     const size_t tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
@@ -381,8 +403,11 @@ void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma) {
   if (count == 0 || ntiles == 0) return;
   if constexpr (std::is_same<T, double>::value) {
-    if (mfma && bm == 128) {
-      gemm_mfma_f64_kernel<<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+    if (mfma) {
+      if (bm == 128)
+        gemm_mfma_f64_kernel<128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+      else
+        gemm_mfma_f64_kernel<64, 128><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
       HODLR_HIP_CHECK(hipGetLastError());
       if (nsplit > 0) {
         dim3 g(64, nsplit);
@@ -404,10 +429,12 @@ void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
   }
 }
 
-template int gemm_tile_m<double>(const std::vector<GemmDesc>&);
-template int gemm_tile_m<float>(const std::vector<GemmDescF>&);
-template size_t gemm_plan<double>(std::vector<GemmDesc>&, std::vector<int>&, int&, int, int);
-template size_t gemm_plan<float>(std::vector<GemmDescF>&, std::vector<int>&, int&, int, int);
+template int gemm_tile_m<double>(const std::vector<GemmDesc>&, bool);
+template int gemm_tile_m<float>(const std::vector<GemmDescF>&, bool);
+template int gemm_tile_n<double>(int, bool);
+template int gemm_tile_n<float>(int, bool);
+template size_t gemm_plan<double>(std::vector<GemmDesc>&, std::vector<int>&, int&, int, int, bool);
+template size_t gemm_plan<float>(std::vector<GemmDescF>&, std::vector<int>&, int&, int, int, bool);
 template void launch_gemm<double>(const GemmDesc*, int, int, double*, const int*, int, int,
                                   hipStream_t, bool);
 template void launch_gemm<float>(const GemmDescF*, int, int, float*, const int*, int, int,

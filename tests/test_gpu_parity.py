@@ -286,10 +286,15 @@ def test_svd_method_gauss1d_4096(gauss1d):
          gemm_mfma=True),
     dict(aca_rows_per_thread=2, gemm_tile_m=128, gemm_split_target=64, lu_panel=64,
          gemm_mfma=True),
+    dict(aca_rows_per_thread=4, gemm_tile_m=64, gemm_split_target=4096, lu_panel=32,
+         gemm_mfma=True),
+    dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=1024, lu_panel=32,
+         gemm_mfma=True),
 ])
 def test_launch_shapes_match_oracle(oracle, shapes):
     """Every launch-shape variant (ACA rows per thread, vector or matrix-core
-    FP64 GEMM tiles, split-K, LU panel width) gives the oracle's ranks and its
+    FP64 GEMM tiles of either shape or chosen per launch, split-K, LU panel
+    width) gives the oracle's ranks and its
     recSM / recLU solutions and log-determinant."""
     _case(f"launch shapes {shapes}")
     cfg = I.config_inputs("gauss1d_4096")
