@@ -124,13 +124,17 @@ void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipSt
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 /* Whole solves of systems of at most LU_SOLVE_SMALL_MAX rows in one launch
- * (interchanges, both substitutions); SolveDesc::chunk0 numbers the
- * workgroups, lu_solve_small_wgs(ncols, maxn) per problem (maxn: the largest
- * system of the launch).  Returns false, and
- * launches nothing, when maxn exceeds the limit. */
+ * (interchanges, both substitutions); SolveDesc::chunk0 numbers the work
+ * units, lu_solve_small_wgs(ncols, maxn, wave) per problem (maxn: the
+ * largest system of the launch; nwg: the units of all problems).  A unit is
+ * a workgroup, or with wave set and maxn <= LU_SOLVE_WAVE_MAX a single wave
+ * (the barrier-free kernel; same results).  Returns false, and launches
+ * nothing, when maxn exceeds the limit. */
 constexpr int LU_SOLVE_SMALL_MAX = 2048;
-int lu_solve_small_wgs(int ncols, int maxn);
-bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
+constexpr int LU_SOLVE_WAVE_MAX = 256;
+int lu_solve_small_wgs(int ncols, int maxn, bool wave);
+bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
+                           hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s);
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);

@@ -764,6 +764,181 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
   }
 }
 
+/* The same solves for systems of at most 64 R rows (R <= 4), one wave per
+ * system and CB columns of B, four independent waves per workgroup: lane i
+ * keeps rows i + 64 r of its columns in registers, and the value of each
+ * step's pivot row is broadcast from its lane with readlane, so a step needs
+ * neither LDS nor a barrier.  The steps run in blocks of D (a multiple of D
+ * never crosses a 64-row group), which keeps the row group of the pivot row
+ * and the slot of the prefetch ring compile-time constants.  The same
+ * arithmetic as lu_solve_small_kernel, operation for operation; the ring
+ * is shorter (D = 4) for four rows per lane, which keeps three waves per
+ * SIMD resident. */
+__device__ __forceinline__ double wave_bcast(double v, int lane) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+template <int R, int CB, int D>
+__global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __restrict__ d,
+                                                            int count, int nunits) {
+  static_assert(64 % D == 0, "step blocks stay inside a row group");
+  const int lane = threadIdx.x & 63;
+  const int unit = (int)blockIdx.x * (TPB / 64) + (int)(threadIdx.x >> 6);
+  if (unit >= nunits) return;  // (whole waves; nothing below synchronizes waves)
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= unit)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SolveDesc p = d[lo];
+  const int n = p.n;
+  const int c0 = (unit - p.chunk0) * CB;
+  const int nc = min(CB, p.ncols - c0);
+  // the interchanges piv[0..n), applied in order, as one permutation: lane i
+  // holds entries i + 64 r
+  int pm[R], pvr[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+    pm[r] = i;
+    pvr[r] = i < n ? p.piv[i] : i;
+  }
+#pragma unroll
+  for (int g = 0; g < R; g++)
+    for (int k = 64 * g; k < min(n, 64 * g + 64); k++) {
+      const int q = __builtin_amdgcn_readlane(pvr[g], k - 64 * g);
+      if (q != k && (unsigned)q < (unsigned)n) {
+        const int a = __builtin_amdgcn_readlane(pm[g], k - 64 * g);
+        int b = a;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (r == (q >> 6)) b = __builtin_amdgcn_readlane(pm[r], q & 63);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = lane + 64 * r;
+          if (i == k) pm[r] = b;
+          if (i == q) pm[r] = a;
+        }
+      }
+    }
+  double x[R][CB];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+#pragma unroll
+    for (int c = 0; c < CB; c++)
+      x[r][c] = (i < n && c < nc) ? p.B[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
+  }
+  const double* LU = p.LU;
+  const size_t ld = (size_t)p.ld_lu;
+  double fr[D][R];
+  // forward: unit lower triangle; fr[s] holds column l (rows below the
+  // diagonal) for the steps l = s (mod D)
+#pragma unroll
+  for (int s = 0; s < D; s++)
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = lane + 64 * r;
+      fr[s][r] = (s < n && i > s && i < n) ? LU[(size_t)i + (size_t)s * ld] : 0.0;
+    }
+#pragma unroll
+  for (int g = 0; g < R; g++)
+    for (int l0 = 64 * g; l0 < min(n, 64 * g + 64); l0 += D) {
+#pragma unroll
+      for (int s = 0; s < D; s++) {
+        const int l = l0 + s;
+        if (l < n) {
+          double xl[CB];
+#pragma unroll
+          for (int c = 0; c < CB; c++) xl[c] = wave_bcast(x[g][c], l - 64 * g);
+#pragma unroll
+          for (int r = g; r < R; r++) {  // rows below l lie in groups g and up
+            const int i = lane + 64 * r;
+            if (i > l && i < n)
+#pragma unroll
+              for (int c = 0; c < CB; c++) x[r][c] -= fr[s][r] * xl[c];
+          }
+          const int ln = l + D;
+#pragma unroll
+          for (int r = 0; r < R; r++) {
+            const int i = lane + 64 * r;
+            fr[s][r] = (ln < n && i > ln && i < n) ? LU[(size_t)i + (size_t)ln * ld] : 0.0;
+          }
+        }
+      }
+    }
+  // backward: upper triangle with its diagonal, from the top step block
+  // (l0 = D - 1 mod D) down; fr[s] holds column l for the steps with
+  // l = D - 1 - s (mod D).  The steps l >= n of the top block only pass
+  // their slot on to column l - D.
+  const int top = (n + D - 1) / D * D - 1;
+#pragma unroll
+  for (int s = 0; s < D; s++) {
+    const int l = top - s;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+      const int i = lane + 64 * r;
+      fr[s][r] = (l >= 0 && l < n && i <= l) ? LU[(size_t)i + (size_t)l * ld] : 0.0;
+    }
+  }
+#pragma unroll
+  for (int g = R - 1; g >= 0; g--)
+    for (int l0 = min(top, 64 * g + 63); l0 >= 64 * g; l0 -= D) {
+#pragma unroll
+      for (int s = 0; s < D; s++) {
+        const int l = l0 - s;
+        if (l < n) {
+          const int own = l - 64 * g;
+          const double dg = wave_bcast(fr[s][g], own);
+          // the owner's CB values to lanes 0..CB-1, one division for all
+          double y = wave_bcast(x[g][0], own);
+#pragma unroll
+          for (int c = 1; c < CB; c++) {
+            const double v = wave_bcast(x[g][c], own);
+            if (lane == c) y = v;
+          }
+          y = y / dg;
+          double xl[CB];
+#pragma unroll
+          for (int c = 0; c < CB; c++) xl[c] = wave_bcast(y, c);
+          if (lane == own)
+#pragma unroll
+            for (int c = 0; c < CB; c++) x[g][c] = xl[c];
+#pragma unroll
+          for (int r = 0; r <= g; r++) {  // rows above l lie in groups g and below
+            const int i = lane + 64 * r;
+            if (i < l)
+#pragma unroll
+              for (int c = 0; c < CB; c++) x[r][c] -= fr[s][r] * xl[c];
+          }
+        }
+        const int ln = l - D;
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = lane + 64 * r;
+          fr[s][r] = (ln >= 0 && ln < n && i <= ln) ? LU[(size_t)i + (size_t)ln * ld] : 0.0;
+        }
+      }
+    }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < CB; c++)
+        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+  }
+}
+
+constexpr int wave_rows(int maxn) { return maxn <= 64 ? 1 : maxn <= 128 ? 2 : 4; }
+constexpr int WAVE_CB = 8;
+
 constexpr int MAX_GRID_Y = 65535;
 
 }  // namespace
@@ -810,14 +985,25 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-int lu_solve_small_wgs(int ncols, int maxn) {
-  const int cb = solve_cols(solve_rows(maxn));
+int lu_solve_small_wgs(int ncols, int maxn, bool wave) {
+  const int cb = wave && maxn <= LU_SOLVE_WAVE_MAX ? WAVE_CB : solve_cols(solve_rows(maxn));
   return (ncols + cb - 1) / cb;
 }
 
-bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s) {
+bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
+                           hipStream_t s) {
   if (maxn > LU_SOLVE_SMALL_MAX) return false;
   if (count == 0 || nwg == 0) return true;
+  if (wave && maxn <= LU_SOLVE_WAVE_MAX) {
+    const int grid = (nwg + TPB / 64 - 1) / (TPB / 64);
+    switch (wave_rows(maxn)) {
+      case 1: lu_solve_wave_kernel<1, WAVE_CB, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      case 2: lu_solve_wave_kernel<2, WAVE_CB, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      default: lu_solve_wave_kernel<4, WAVE_CB, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+    }
+    HODLR_HIP_CHECK(hipGetLastError());
+    return true;
+  }
   switch (solve_rows(maxn)) {
     case 1: lu_solve_small_kernel<1><<<nwg, TPB, 0, s>>>(d, count); break;
     case 2: lu_solve_small_kernel<2><<<nwg, TPB, 0, s>>>(d, count); break;

@@ -72,6 +72,9 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   // blocked path on small problems)
   if (const char* e = std::getenv("HODLR_LU_SOLVE_SMALL_MAX"))
     solve_small_max_ = std::max(0, std::min(LU_SOLVE_SMALL_MAX, std::atoi(e)));
+  // and the one-wave-per-system kernel (0: the workgroup kernel for all)
+  if (const char* e = std::getenv("HODLR_LU_SOLVE_WAVE_MAX"))
+    solve_wave_max_ = std::max(0, std::min(LU_SOLVE_WAVE_MAX, std::atoi(e)));
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreate(&ev0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
@@ -493,12 +496,14 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
   if (maxn <= solve_small_max_) {  // every system in one launch
+    const bool wave = maxn <= solve_wave_max_;
     int nwg = 0;
     for (auto& d : descs) {
       d.chunk0 = nwg;
-      nwg += lu_solve_small_wgs(d.ncols, maxn);
+      nwg += lu_solve_small_wgs(d.ncols, maxn, wave);
     }
-    launch_lu_solve_small(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, stream_);
+    launch_lu_solve_small(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, wave,
+                          stream_);
     return;
   }
   {

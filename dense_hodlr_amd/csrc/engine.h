@@ -292,6 +292,7 @@ class Engine {
   std::vector<double> pred_rank_;
   bool trace_ = false;
   int solve_small_max_ = LU_SOLVE_SMALL_MAX;  // one-launch LU solves up to this size
+  int solve_wave_max_ = LU_SOLVE_WAVE_MAX;    // of which one wave per system up to this size
   std::vector<hipEvent_t> prof_ev_;
 
   hipStream_t stream_ = nullptr;

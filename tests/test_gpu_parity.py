@@ -331,6 +331,31 @@ def test_blocked_lu_solves_match_oracle(oracle, monkeypatch, limit):
     assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-11 * abs(o.logAbsDeterminant())
 
 
+@pytest.mark.parametrize("leaf", [40, 100, 200])
+def test_wave_lu_solves_match_workgroup_kernel(monkeypatch, leaf):
+    """Systems of at most 256 rows are solved one wave per system and column
+    chunk (no barriers, broadcasts by readlane); the same operations as the
+    workgroup kernel, so recSM, recLU and the log-det are bit-identical with
+    the wave kernel switched off.  Leaf sizes 40/100/200 (ragged trees of
+    3000 points) put the leaves and node systems on one, two and four rows
+    per lane; the oracle tests above run the wave kernel (the default)."""
+    _case(f"wave LU solves vs workgroup kernel, leaf {leaf}")
+    n = 3000
+    pts = I.points_2d_kd(n, 41)
+    b = I.rhs_normal(n, 5, 42)
+    out = []
+    for wave_max in ("256", "0"):
+        monkeypatch.setenv("HODLR_LU_SOLVE_WAVE_MAX", wave_max)
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.08, 1e-3, sizeThreshold=leaf)
+        g.set_LRTolerance(1e-10)
+        out.append((g.recSM_Solve(b), g.recLU_Solve(b), g.logAbsDeterminant(), g.nodes()))
+        del g
+    (xs0, xl0, d0, n0), (xs1, xl1, d1, n1) = out
+    assert np.array_equal(n0, n1)
+    assert np.array_equal(xs0, xs1) and np.array_equal(xl0, xl1)
+    assert d0 == d1
+
+
 def test_matern2d_65536_against_oracle_summary():
     """The N=65536 Matern-3/2 workload at full size (16 right-hand sides, the
     log-determinant) against the oracle's recSM on the same seeded inputs,
