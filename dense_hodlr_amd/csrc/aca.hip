@@ -40,6 +40,7 @@ namespace {
 
 constexpr int TPB = 256;   // threads per workgroup
 constexpr int LCH = 16;    // l-chunk of the transposed partial reduction
+constexpr int DOT_WIN = 128;  // columns whose dot products are combined per workgroup barrier
 constexpr int FIN_L = 64;  // dot products per finalize workgroup (x4 slot groups)
 
 __device__ inline void argmax_merge(double& bv, int& bi, double v, int i) {
@@ -169,14 +170,23 @@ __device__ __forceinline__ double add_mul_rn(double s, double w, double v) {
  *   RES:  s[q] = sum_l bc[l*ld] * P(idx[q], l), in order of l with separately
  *         rounded multiply and add (the reference's residual loop);
  *   dots: out[l] = workgroup sum of sum_q P(idx[q], l) * pv[q]  (l < k).
- * Full chunks of LCH columns are unrolled so all their loads issue first. */
+ * Full chunks of LCH columns are unrolled so all their loads issue first.
+ *
+ * The dot products are summed without a workgroup barrier per chunk: each
+ * wave transposes its threads' products through its own columns of red
+ * (a wave runs in lockstep, so its LDS writes are visible to its own later
+ * reads) into sixteen part sums of sixteen threads each, kept in wsum; every
+ * DOT_WIN columns one barrier pair combines the parts of the four waves in a
+ * fixed tree.  Parts and tree are those of a 256-thread transposed
+ * reduction, so the sums do not depend on the window. */
 template <int JP, bool RES>
 __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
                                            const double* __restrict__ bc, int ld, int k,
                                            const int (&idx)[JP], const double (&pv)[JP],
                                            bool dots, double (&s)[JP], double (*red)[TPB],
-                                           double* out) {
-  const int t = threadIdx.x;
+                                           double (*wsum)[DOT_WIN], double* out) {
+  static_assert(TPB == 256 && LCH == 16, "sixteen parts of sixteen threads per column");
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
   for (int l0 = 0; l0 < k; l0 += LCH) {
     const int lc = min(LCH, k - l0);
     if (lc == LCH) {
@@ -212,16 +222,29 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
       }
     }
     if (dots) {
-      __syncthreads();
-      const int ll = t >> 4, part = t & 15;
-      double sum = 0.0;
+      __builtin_amdgcn_wave_barrier();
+      // part 4 wave + pp of column ll: threads 64 wave + 16 pp + u, u < 16
+      const int ll = lane >> 2, pp = lane & 3;
       if (ll < lc) {
+        double sum = 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; u++) sum += red[ll][part * 16 + u];
+        for (int u = 0; u < 16; u++) sum += red[ll][wave * 64 + pp * 16 + u];
+        wsum[wave * 4 + pp][l0 % DOT_WIN + ll] = sum;
       }
-      for (int off = 8; off > 0; off >>= 1) sum += __shfl_xor(sum, off);
-      if (part == 0 && ll < lc) out[l0 + ll] = sum;
-      __syncthreads();
+      __builtin_amdgcn_wave_barrier();
+      if ((l0 + LCH) % DOT_WIN == 0 || l0 + LCH >= k) {
+        __syncthreads();
+        const int wb = l0 / DOT_WIN * DOT_WIN, we = l0 + lc;
+        if (wb + t < we) {
+          const int x = t;
+          double a[8];
+#pragma unroll
+          for (int q = 0; q < 8; q++) a[q] = wsum[q][x] + wsum[q + 8][x];
+          // the xor tree over parts (offsets 8, 4, 2, 1) as seen by part 0
+          out[wb + x] = ((a[0] + a[4]) + (a[2] + a[6])) + ((a[1] + a[5]) + (a[3] + a[7]));
+        }
+        __syncthreads();
+      }
     }
   }
 }
@@ -251,6 +274,7 @@ __global__ void __launch_bounds__(TPB)
                    const AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
   __shared__ double red[LCH][TPB];
+  __shared__ double wsum[16][DOT_WIN];
   __shared__ PivRed pr_sh[TPB / 64];
   const int bidx = wg2blk[blockIdx.x];
   const AcaBlock b = blocks[bidx];
@@ -280,7 +304,7 @@ __global__ void __launch_bounds__(TPB)
   }
   double* slot = slot_ptr(L, b, local);
   // residual row and, for the pending term k-1, the dots V(:,l) . v_{k-1}
-  panel_pass<JPT, true>(Vp, Wp + i, ld, k, jj, pv, pend, sacc, red, slot);
+  panel_pass<JPT, true>(Vp, Wp + i, ld, k, jj, pv, pend, sacc, red, wsum, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
@@ -312,6 +336,7 @@ __global__ void __launch_bounds__(TPB)
                    AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
   __shared__ double red[LCH][TPB];
+  __shared__ double wsum[16][DOT_WIN];
   __shared__ PivRed pr_sh[TPB / 64];
   const int bidx = wg2blk[blockIdx.x];
   const AcaBlock b = blocks[bidx];
@@ -346,7 +371,7 @@ __global__ void __launch_bounds__(TPB)
       if (!ok) ii[q] = -1;
       sacc[q] = 0.0;
     }
-    panel_pass<IPT, false>(Wp, nullptr, ld, k, ii, pw, true, sacc, red, slot);
+    panel_pass<IPT, false>(Wp, nullptr, ld, k, ii, pw, true, sacc, red, wsum, slot);
     return;
   }
   const SRC src = srcs[b.inst];
@@ -361,7 +386,7 @@ __global__ void __launch_bounds__(TPB)
     if (!ok) ii[q] = -1;
     sacc[q] = 0.0;
   }
-  panel_pass<IPT, true>(Wp, Vp + j, ld, k, ii, pw, pend, sacc, red, slot);
+  panel_pass<IPT, true>(Wp, Vp + j, ld, k, ii, pw, pend, sacc, red, wsum, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
