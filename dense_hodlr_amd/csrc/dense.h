@@ -16,6 +16,24 @@ struct LuDesc {
   int* piv;  /* 0-based row interchanges, LAPACK style */
   int n;
   int ld;
+  /* LU_SWP_INTS ints of scratch: the rows the latest panel's interchanges
+   * move in the columns outside it (destinations at [0, 2 pw), their
+   * sources at [LU_SWP_SRC, LU_SWP_SRC + 2 pw)), for lu_step_kernel */
+  int* swp = nullptr;
+};
+constexpr int LU_SWP_SRC = 128;
+constexpr int LU_SWP_INTS = 256;
+
+/* One blocked-LU step outside the panel [p0, p0 + pw) of a matrix: the
+ * panel's row interchanges for the columns left of it (chunks [0, nleft))
+ * and right of it, and for the right ones the triangular solve
+ * U12 = L11^-1 A12.  64 columns per workgroup. */
+struct LuStepDesc {
+  double* A;
+  const int* swp;
+  int n, ld, p0, pw;
+  int chunk0; /* first workgroup of this matrix */
+  int nleft;  /* workgroups of the columns left of the panel */
 };
 
 /* In-place solve A X = B with A given by an LuDesc; B is n x ncols. */
@@ -121,6 +139,12 @@ struct PermDesc {
  * the global-memory kernel beyond 4096 rows). */
 int lu_panel_width(int maxn, int nb);
 void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipStream_t s);
+/* Whether the panel kernel for (maxn, nb) leaves the interchanges of the
+ * columns outside the panel to launch_lu_step (register panels; the
+ * global-memory panel kernel swaps whole rows itself). */
+bool lu_panel_defers_swaps(int maxn, int nb);
+/* swap: apply the interchanges recorded in LuDesc::swp (see above). */
+void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s);
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 /* Whole solves of systems of at most LU_SOLVE_SMALL_MAX rows in one launch

@@ -199,8 +199,9 @@ __global__ void __launch_bounds__(TPB) lu_panel_kernel(const LuDesc* __restrict_
  * Per column: first-max pivot search over the owned rows (wave shuffles,
  * then LDS), exchange of the two rows through LDS, broadcast of the pivot
  * row, scaling and rank-1 update in registers.  The row interchanges of the
- * columns outside the panel are applied afterwards, in the same order, one
- * thread per column.  Used when the largest matrix of the batch has at most
+ * columns outside the panel are recorded for lu_step_kernel (a row of a
+ * column-major matrix is n separate cache lines: one workgroup swapping
+ * them column by column took half the panel step).  Used when the largest matrix of the batch has at most
  * NT * RPT rows. */
 template <int NT, int RPT, int PW>
 __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restrict__ d, int p0,
@@ -214,6 +215,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
   __shared__ double prow[2][PW];  // old row p: the pivot row after the exchange
   __shared__ double crow[2][PW];  // old row cr: moves to row p
   __shared__ int piv_sh[PW];
+  __shared__ int perm[NT * RPT];
   const LuDesc m = d[blockIdx.x];
   const int n = m.n, ld = m.ld, t = threadIdx.x;
   if (n <= p0) return;
@@ -330,19 +332,83 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
         if (j < pw) A[(size_t)r + (size_t)(p0 + j) * ld] = R[q][j];
     }
   }
-  // the same interchanges, in order, for the columns outside the panel
-  for (int j = t; j < n; j += NT) {
-    if (j >= p0 && j < p0 + pw) continue;
-    double* col = A + (size_t)j * ld;
+  // The interchanges of the columns outside the panel are applied by
+  // lu_step_kernel, many workgroups wide: here they are composed into one
+  // permutation (row p0 + o then holds the old row p0 + perm[o]), and the
+  // rows it moves are recorded: the panel rows and the pivot rows, as
+  // destinations with their sources (a pivot row may repeat: same source).
+  for (int o = t; o < n - p0; o += NT) perm[o] = o;
+  __syncthreads();
+  if (t == 0)
     for (int c = 0; c < pw; c++) {
-      const int p = piv_sh[c];
-      if (p != p0 + c) {
-        const double x = col[p0 + c];
-        col[p0 + c] = col[p];
-        col[p] = x;
+      const int q = piv_sh[c] - p0;
+      if (q != c) {
+        const int x = perm[c];
+        perm[c] = perm[q];
+        perm[q] = x;
       }
     }
+  __syncthreads();
+  if (t < 2 * pw) {
+    const int dr = t < pw ? p0 + t : piv_sh[t - pw];
+    m.swp[t] = dr;
+    m.swp[LU_SWP_SRC + t] = p0 + perm[dr - p0];
   }
+}
+
+/* The blocked LU's work outside the panel [p0, p0 + pw), one column per
+ * thread, 64 per workgroup: the panel's interchanges (the recorded 2 pw
+ * moves: all loads, then all stores), and for the columns right of the
+ * panel U12 = L11^-1 A12 by forward substitution, operation for operation
+ * as trsm_kernel. */
+__global__ void __launch_bounds__(64) lu_step_kernel(const LuStepDesc* __restrict__ d, int count,
+                                                      int swap) {
+  __shared__ double T[64][65];
+  __shared__ double X[64][65];
+  __shared__ int sd[2 * 32], ss[2 * 32];
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const LuStepDesc p = d[lo];
+  const int t = threadIdx.x, pw = p.pw, p0 = p.p0;
+  const int ch = (int)blockIdx.x - p.chunk0;
+  const bool left = ch < p.nleft;
+  const int j = left ? ch * 64 + t : p0 + pw + (ch - p.nleft) * 64 + t;
+  const bool ok = left ? j < p0 : j < p.n;
+  const int nm = swap ? 2 * pw : 0;  // (recorded only by the register panels: pw <= 32)
+  for (int u = t; u < nm; u += 64) {
+    sd[u] = p.swp[u];
+    ss[u] = p.swp[LU_SWP_SRC + u];
+  }
+  if (!left)
+    for (int e = t; e < pw * pw; e += 64) {
+      const int r = e % pw, c = e / pw;
+      T[r][c] = p.A[(size_t)(p0 + r) + (size_t)(p0 + c) * p.ld];
+    }
+  __syncthreads();
+  double* col = p.A + (size_t)j * p.ld;
+  if (ok && nm > 0) {
+#pragma unroll 8
+    for (int u = 0; u < nm; u++)
+      if (ss[u] != sd[u]) X[u][t] = col[ss[u]];
+#pragma unroll 8
+    for (int u = 0; u < nm; u++)
+      if (ss[u] != sd[u]) col[sd[u]] = X[u][t];
+  }
+  if (left) return;
+  for (int r = 0; r < pw; r++) X[r][t] = ok ? col[p0 + r] : 0.0;
+  for (int r = 1; r < pw; r++) {
+    double v = X[r][t];
+    for (int q = 0; q < r; q++) v -= T[r][q] * X[q][t];
+    X[r][t] = v;
+  }
+  if (ok)
+    for (int r = 0; r < pw; r++) col[p0 + r] = X[r][t];
 }
 
 /* Triangular solve of a pw x pw diagonal block (pw <= 64) on a chunk of 64
@@ -976,6 +1042,17 @@ void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipSt
     lu_panel_reg_kernel<512, 8, 4><<<count, 512, 0, s>>>(d, p0, nb);
   else
     lu_panel_kernel<<<count, TPB, 0, s>>>(d, p0, nb);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+bool lu_panel_defers_swaps(int maxn, int nb) {
+  return (maxn <= 512 && nb <= 32) || (maxn <= 1024 && nb <= 16) || (maxn <= 2048 && nb <= 8) ||
+         (maxn <= 4096 && nb <= 4);
+}
+
+void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s) {
+  if (count == 0 || nwg == 0) return;
+  lu_step_kernel<<<nwg, 64, 0, s>>>(d, count, swap ? 1 : 0);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

@@ -416,25 +416,30 @@ static GemmDescT<T> gemm_desc(int M, int N, int K, int ta, const T* A, int lda, 
 }
 
 /* Blocked right-looking LU with partial pivoting of a batch of matrices
- * (getrf semantics): per panel of LU_NB columns, the panel factorization with
- * whole-row swaps, the triangular solve for U12 and the trailing update
- * A22 -= L21 U12 as one grouped GEMM over the batch. */
+ * (getrf semantics): per panel of LU_NB columns, the panel factorization,
+ * the step kernel (the panel's row interchanges in the columns outside it,
+ * the triangular solve for U12) and the trailing update A22 -= L21 U12 as
+ * one grouped GEMM over the batch. */
 void Engine::lu_batch(std::vector<LuDesc>& descs) {
   if (descs.empty()) return;
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
-  const LuDesc* dd = arena_.push(descs, stream_);
   const int nb = lu_panel_width(maxn, tune_.lu_nb);
+  // the register panels record their interchanges for the step kernel
+  const bool defer = lu_panel_defers_swaps(maxn, nb);
+  grow_t(lu_swp_, descs.size() * LU_SWP_INTS);
+  for (size_t i = 0; i < descs.size(); i++) descs[i].swp = lu_swp_.p + i * LU_SWP_INTS;
+  const LuDesc* dd = arena_.push(descs, stream_);
   // Every panel step's TRSM and trailing-update descriptors are planned on
   // the host first and uploaded together: one copy per batch instead of two
   // or three per panel (a 1090-row matrix has 137 panels of 8).
   struct Step {
-    int tr0, ntr, nwg;            // TRSM descriptors [tr0, tr0 + ntr)
+    int tr0, ntr, nwg;            // step descriptors [tr0, tr0 + ntr)
     int g0, ng, ntiles, bm;       // GEMM descriptors [g0, g0 + ng)
     int id0, nid;                 // split-K problem ids [id0, id0 + nid)
   };
   std::vector<Step> steps;
-  std::vector<TrsmDesc> tr_all;
+  std::vector<LuStepDesc> tr_all;
   std::vector<GemmDesc> g_all;
   std::vector<int> id_all;
   size_t ws = 0;
@@ -444,18 +449,22 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
     for (auto& d : descs) {
       const int pw = std::min(nb, d.n - p0);
       const int rest = d.n - p0 - pw;
-      if (pw <= 0 || rest <= 0) continue;
-      TrsmDesc t{};
-      t.T = d.A + p0 + (size_t)p0 * d.ld;
-      t.B = d.A + p0 + (size_t)(p0 + pw) * d.ld;
-      t.ldt = d.ld;
-      t.ldb = d.ld;
-      t.pw = pw;
-      t.upper = 0;
-      t.ncols = rest;
-      t.chunk0 = st.nwg;
-      st.nwg += (rest + 63) / 64;
-      tr_all.push_back(t);
+      if (pw <= 0) continue;
+      const int nleft = defer ? (p0 + 63) / 64 : 0, nright = (rest + 63) / 64;
+      if (nleft + nright > 0) {
+        LuStepDesc t{};
+        t.A = d.A;
+        t.swp = d.swp;
+        t.n = d.n;
+        t.ld = d.ld;
+        t.p0 = p0;
+        t.pw = pw;
+        t.chunk0 = st.nwg;
+        t.nleft = nleft;
+        st.nwg += nleft + nright;
+        tr_all.push_back(t);
+      }
+      if (rest <= 0) continue;
       g.push_back(gemm_desc(rest, rest, pw, 0, d.A + (p0 + pw) + (size_t)p0 * d.ld, d.ld,
                             d.A + p0 + (size_t)(p0 + pw) * d.ld, d.ld,
                             d.A + (p0 + pw) + (size_t)(p0 + pw) * d.ld, d.ld, -1.0, 1.0));
@@ -474,17 +483,17 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
     steps.push_back(st);
   }
   if (ws) ensure_ws(ws);
-  const TrsmDesc* dtr = tr_all.empty() ? nullptr : arena_.push(tr_all, stream_);
+  const LuStepDesc* dtr = tr_all.empty() ? nullptr : arena_.push(tr_all, stream_);
   const GemmDesc* dg = g_all.empty() ? nullptr : arena_.push(g_all, stream_);
   const int* did = id_all.empty() ? nullptr : arena_.push(id_all, stream_);
   int p = 0;
   for (int p0 = 0; p0 < maxn; p0 += nb, p++) {
     launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, stream_);
     const Step& st = steps[p];
-    if (st.ntr == 0) continue;
-    launch_trsm(dtr + st.tr0, st.ntr, st.nwg, stream_);
-    launch_gemm<double>(dg + st.g0, st.ng, st.ntiles, ws_.p, did ? did + st.id0 : nullptr, st.nid,
-                        st.bm, stream_, tune_.gemm_mfma != 0);
+    if (st.ntr > 0) launch_lu_step(dtr + st.tr0, st.ntr, st.nwg, defer, stream_);
+    if (st.ng > 0)
+      launch_gemm<double>(dg + st.g0, st.ng, st.ntiles, ws_.p, did ? did + st.id0 : nullptr,
+                          st.nid, st.bm, stream_, tune_.gemm_mfma != 0);
   }
 }
 

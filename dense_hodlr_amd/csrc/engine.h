@@ -310,6 +310,7 @@ class Engine {
   int ktot_ = 0;
   DBuf<double> Lorig_, Lfac_, Linv_;
   DBuf<int> Lpiv_;
+  DBuf<int> lu_swp_;  // the batched LU's recorded panel interchanges (LU_SWP_INTS per matrix)
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;
   Factorization fac_[2];
   MixedFactorization mfac_[2];
