@@ -356,13 +356,16 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
   }
 }
 
-/* The blocked LU's work outside the panel [p0, p0 + pw), one column per
- * thread, 64 per workgroup: the panel's interchanges (the recorded 2 pw
- * moves: all loads, then all stores), and for the columns right of the
- * panel U12 = L11^-1 A12 by forward substitution, operation for operation
- * as trsm_kernel. */
-__global__ void __launch_bounds__(64) lu_step_kernel(const LuStepDesc* __restrict__ d, int count,
-                                                      int swap) {
+/* The blocked LU's work outside the panel [p0, p0 + pw), 64 columns per
+ * workgroup of 256 threads: the panel's interchanges (the recorded 2 pw
+ * moves of each column, spread over all four waves: all loads, a barrier,
+ * all stores), then for the columns right of the panel U12 = L11^-1 A12 by
+ * forward substitution in the first wave, one column per thread, operation
+ * for operation as trsm_kernel. */
+constexpr int LU_STEP_NT = 256;
+
+__global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* __restrict__ d,
+                                                              int count, int swap) {
   __shared__ double T[64][65];
   __shared__ double X[64][65];
   __shared__ int sd[2 * 32], ss[2 * 32];
@@ -378,29 +381,37 @@ __global__ void __launch_bounds__(64) lu_step_kernel(const LuStepDesc* __restric
   const int t = threadIdx.x, pw = p.pw, p0 = p.p0;
   const int ch = (int)blockIdx.x - p.chunk0;
   const bool left = ch < p.nleft;
-  const int j = left ? ch * 64 + t : p0 + pw + (ch - p.nleft) * 64 + t;
-  const bool ok = left ? j < p0 : j < p.n;
+  const int j0 = left ? ch * 64 : p0 + pw + (ch - p.nleft) * 64;  // first column of the chunk
+  const int jend = left ? p0 : p.n;
   const int nm = swap ? 2 * pw : 0;  // (recorded only by the register panels: pw <= 32)
-  for (int u = t; u < nm; u += 64) {
+  for (int u = t; u < nm; u += LU_STEP_NT) {
     sd[u] = p.swp[u];
     ss[u] = p.swp[LU_SWP_SRC + u];
   }
   if (!left)
-    for (int e = t; e < pw * pw; e += 64) {
+    for (int e = t; e < pw * pw; e += LU_STEP_NT) {
       const int r = e % pw, c = e / pw;
       T[r][c] = p.A[(size_t)(p0 + r) + (size_t)(p0 + c) * p.ld];
     }
   __syncthreads();
-  double* col = p.A + (size_t)j * p.ld;
-  if (ok && nm > 0) {
-#pragma unroll 8
-    for (int u = 0; u < nm; u++)
-      if (ss[u] != sd[u]) X[u][t] = col[ss[u]];
-#pragma unroll 8
-    for (int u = 0; u < nm; u++)
-      if (ss[u] != sd[u]) col[sd[u]] = X[u][t];
+  if (nm > 0) {
+    for (int e = t; e < nm * 64; e += LU_STEP_NT) {
+      const int u = e >> 6, c = e & 63;
+      if (j0 + c < jend && ss[u] != sd[u])
+        X[u][c] = p.A[(size_t)ss[u] + (size_t)(j0 + c) * p.ld];
+    }
+    __syncthreads();
+    for (int e = t; e < nm * 64; e += LU_STEP_NT) {
+      const int u = e >> 6, c = e & 63;
+      if (j0 + c < jend && ss[u] != sd[u])
+        p.A[(size_t)sd[u] + (size_t)(j0 + c) * p.ld] = X[u][c];
+    }
+    __syncthreads();  // the substitution below reads rows the stores wrote
   }
-  if (left) return;
+  if (left || t >= 64) return;
+  const int j = j0 + t;
+  const bool ok = j < jend;
+  double* col = p.A + (size_t)j * p.ld;
   for (int r = 0; r < pw; r++) X[r][t] = ok ? col[p0 + r] : 0.0;
   for (int r = 1; r < pw; r++) {
     double v = X[r][t];
@@ -1052,7 +1063,7 @@ bool lu_panel_defers_swaps(int maxn, int nb) {
 
 void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
-  lu_step_kernel<<<nwg, 64, 0, s>>>(d, count, swap ? 1 : 0);
+  lu_step_kernel<<<nwg, LU_STEP_NT, 0, s>>>(d, count, swap ? 1 : 0);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
