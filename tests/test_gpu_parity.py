@@ -125,6 +125,33 @@ def test_exact_hodlr_dense_constructor(oracle):
     assert np.array_equal(g.nodes(), o.nodes())
 
 
+@pytest.mark.parametrize("n", [3000, 4500])
+def test_single_large_leaf_lu_paths(n):
+    """One dense leaf of n rows (sizeThreshold above n): the batched LU at
+    n = 3000 runs 4-column register panels whose interchanges the step
+    kernel applies to the columns left and right of each panel; at n = 4500
+    the global-memory panel kernel swaps whole rows itself.  A random dense
+    matrix pivots at almost every column.  Solutions against LAPACK's
+    (numpy.linalg.solve) within cond(A) x 1e-13, the normwise residual
+    within 1e-12 (the leaf solve multiplies by the explicit inverse, whose
+    residual is not LAPACK's few ulps; measured 1.3e-14 at n = 3000),
+    log|det| against slogdet."""
+    _case(f"single leaf n={n}")
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, 2))
+    g = HODLR_Matrix(A, n + 1, "partialPiv_ACA")
+    xs = g.recSM_Solve(b)
+    xl = g.recLU_Solve(b)
+    xr = np.linalg.solve(A, b)
+    sign, ld = np.linalg.slogdet(A)
+    cond = np.linalg.cond(A)
+    print(f"cond {cond:.2e} rel diff recSM {rel(xs, xr):.2e} recLU {rel(xl, xr):.2e}", flush=True)
+    assert rel(xs, xr) < 1e-13 * cond and rel(xl, xr) < 1e-13 * cond
+    assert np.linalg.norm(A @ xs - b) / (np.linalg.norm(A, 2) * np.linalg.norm(xs)) < 1e-12
+    assert abs(g.logAbsDeterminant() - ld) < 1e-10 * abs(ld)
+
+
 def test_radial_imq_replace_diagonal(oracle):
     # test/test.cpp:112-148 at n = 3000: IMQ, diagonal 0, leaf 30, tol 1e-8, < 1e-6
     _case("IMQ radial 3000")
