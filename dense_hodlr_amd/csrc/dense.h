@@ -155,7 +155,7 @@ void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
  * (the barrier-free kernel; same results).  Returns false, and launches
  * nothing, when maxn exceeds the limit. */
 constexpr int LU_SOLVE_SMALL_MAX = 2048;
-constexpr int LU_SOLVE_WAVE_MAX = 256;
+constexpr int LU_SOLVE_WAVE_MAX = 512;
 int lu_solve_small_wgs(int ncols, int maxn, bool wave);
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
                            hipStream_t s);

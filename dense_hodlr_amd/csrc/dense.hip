@@ -841,7 +841,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
   }
 }
 
-/* The same solves for systems of at most 64 R rows (R <= 4), one wave per
+/* The same solves for systems of at most 64 R rows (R <= 8), one wave per
  * system and CB columns of B, four independent waves per workgroup: lane i
  * keeps rows i + 64 r of its columns in registers, and the value of each
  * step's pivot row is broadcast from its lane with readlane, so a step needs
@@ -849,8 +849,8 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
  * never crosses a 64-row group), which keeps the row group of the pivot row
  * and the slot of the prefetch ring compile-time constants.  The same
  * arithmetic as lu_solve_small_kernel, operation for operation; the ring
- * is shorter (D = 4) for four rows per lane, which keeps three waves per
- * SIMD resident. */
+ * is shorter (D = 4) for four or eight rows per lane, which keeps two or
+ * three waves per SIMD resident. */
 __device__ __forceinline__ double wave_bcast(double v, int lane) {
   const long long b = __double_as_longlong(v);
   const int lo = __builtin_amdgcn_readlane((int)b, lane);
@@ -1013,8 +1013,11 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
   }
 }
 
-constexpr int wave_rows(int maxn) { return maxn <= 64 ? 1 : maxn <= 128 ? 2 : 4; }
-constexpr int WAVE_CB = 8;
+constexpr int wave_rows(int maxn) { return maxn <= 64 ? 1 : maxn <= 128 ? 2 : maxn <= 256 ? 4 : 8; }
+// columns per wave: fewer for more rows per lane (registers: R x CB values
+// and the D x R ring; beyond 8 rows per lane the compiler no longer keeps
+// the unrolled arrays in registers)
+constexpr int wave_cols(int R) { return R <= 4 ? 8 : 4; }
 
 constexpr int MAX_GRID_Y = 65535;
 
@@ -1074,7 +1077,8 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
 }
 
 int lu_solve_small_wgs(int ncols, int maxn, bool wave) {
-  const int cb = wave && maxn <= LU_SOLVE_WAVE_MAX ? WAVE_CB : solve_cols(solve_rows(maxn));
+  const int cb =
+      wave && maxn <= LU_SOLVE_WAVE_MAX ? wave_cols(wave_rows(maxn)) : solve_cols(solve_rows(maxn));
   return (ncols + cb - 1) / cb;
 }
 
@@ -1085,9 +1089,10 @@ bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, boo
   if (wave && maxn <= LU_SOLVE_WAVE_MAX) {
     const int grid = (nwg + TPB / 64 - 1) / (TPB / 64);
     switch (wave_rows(maxn)) {
-      case 1: lu_solve_wave_kernel<1, WAVE_CB, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
-      case 2: lu_solve_wave_kernel<2, WAVE_CB, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
-      default: lu_solve_wave_kernel<4, WAVE_CB, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      case 1: lu_solve_wave_kernel<1, 8, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      case 2: lu_solve_wave_kernel<2, 8, 8><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      case 4: lu_solve_wave_kernel<4, 8, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
+      default: lu_solve_wave_kernel<8, 4, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
     }
     HODLR_HIP_CHECK(hipGetLastError());
     return true;

@@ -358,14 +358,15 @@ def test_blocked_lu_solves_match_oracle(oracle, monkeypatch, limit):
     assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-11 * abs(o.logAbsDeterminant())
 
 
-@pytest.mark.parametrize("leaf", [40, 100, 200])
+@pytest.mark.parametrize("leaf", [40, 100, 200, 400])
 def test_wave_lu_solves_match_workgroup_kernel(monkeypatch, leaf):
-    """Systems of at most 256 rows are solved one wave per system and column
+    """Systems of at most 512 rows are solved one wave per system and column
     chunk (no barriers, broadcasts by readlane); the same operations as the
     workgroup kernel, so recSM, recLU and the log-det are bit-identical with
-    the wave kernel switched off.  Leaf sizes 40/100/200 (ragged trees of
-    3000 points) put the leaves and node systems on one, two and four rows
-    per lane; the oracle tests above run the wave kernel (the default)."""
+    the wave kernel switched off.  Leaf sizes 40/100/200/400 (ragged trees
+    of 3000 points) put the leaves and node systems on one, two, four and
+    eight rows per lane; the oracle tests above run the wave kernel (the
+    default)."""
     _case(f"wave LU solves vs workgroup kernel, leaf {leaf}")
     n = 3000
     pts = I.points_2d_kd(n, 41)
