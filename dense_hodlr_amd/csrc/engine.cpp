@@ -78,6 +78,11 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreate(&ev0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
+  HODLR_HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
+  HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_cap_, hipEventDisableTiming));
+  HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_root_, hipEventDisableTiming));
+  // (off: the roots' LU on the engine's stream, as every other level's)
+  overlap_root_ = std::getenv("HODLR_NO_ROOT_OVERLAP") == nullptr;
   build_trees();
   upload_sources();
   // problem shape: instance sizes, leaf sizes and kinds
@@ -155,11 +160,28 @@ void Engine::autotune() {
 }
 
 Engine::~Engine() {
+  if (aux_) (void)hipStreamSynchronize(aux_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto e : prof_ev_) (void)hipEventDestroy(e);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
+  if (ev_cap_) (void)hipEventDestroy(ev_cap_);
+  if (ev_root_) (void)hipEventDestroy(ev_root_);
+  if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void Engine::wait_root(Factorization& F) {
+  if (F.root_pending) HODLR_HIP_CHECK(hipStreamWaitEvent(stream_, ev_root_, 0));
+}
+
+void Engine::finish_root() {
+  bool any = false;
+  for (auto& f : fac_) any = any || f.root_pending;
+  if (!any) return;
+  HODLR_HIP_CHECK(hipEventSynchronize(ev_root_));
+  for (auto& f : fac_) f.root_pending = false;
+  aux_arena_.reset();
 }
 
 /* Default tree of each instance (src/HODLR_Tree.cpp:33-132): the root splits
@@ -279,6 +301,7 @@ void Engine::upload_sources() {
 /* settings (reset_attributes semantics, HODLR_Matrix.cpp:1232-1286)    */
 /* ------------------------------------------------------------------ */
 void Engine::reset_attributes() {
+  finish_root();
   buf_gen_++;
   graph_.release();
   compressed_ = false;
@@ -421,6 +444,11 @@ static GemmDescT<T> gemm_desc(int M, int N, int K, int ta, const T* A, int lda, 
  * the triangular solve for U12) and the trailing update A22 -= L21 U12 as
  * one grouped GEMM over the batch. */
 void Engine::lu_batch(std::vector<LuDesc>& descs) {
+  finish_root();  // (the side-stream LU shares the interchange scratch)
+  lu_batch(descs, stream_, arena_);
+}
+
+void Engine::lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& arena) {
   if (descs.empty()) return;
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
@@ -429,7 +457,7 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
   const bool defer = lu_panel_defers_swaps(maxn, nb);
   grow_t(lu_swp_, descs.size() * LU_SWP_INTS);
   for (size_t i = 0; i < descs.size(); i++) descs[i].swp = lu_swp_.p + i * LU_SWP_INTS;
-  const LuDesc* dd = arena_.push(descs, stream_);
+  const LuDesc* dd = arena.push(descs, s);
   // Every panel step's TRSM and trailing-update descriptors are planned on
   // the host first and uploaded together: one copy per batch instead of two
   // or three per panel (a 1090-row matrix has 137 panels of 8).
@@ -482,18 +510,21 @@ void Engine::lu_batch(std::vector<LuDesc>& descs) {
     }
     steps.push_back(st);
   }
+  // (the panel updates have K = panel width <= 64: never split, no workspace;
+  // ensure_ws would reallocate a buffer another stream may be using)
+  if (ws && s != stream_) throw std::logic_error("split-K update in a side-stream LU");
   if (ws) ensure_ws(ws);
-  const LuStepDesc* dtr = tr_all.empty() ? nullptr : arena_.push(tr_all, stream_);
-  const GemmDesc* dg = g_all.empty() ? nullptr : arena_.push(g_all, stream_);
-  const int* did = id_all.empty() ? nullptr : arena_.push(id_all, stream_);
+  const LuStepDesc* dtr = tr_all.empty() ? nullptr : arena.push(tr_all, s);
+  const GemmDesc* dg = g_all.empty() ? nullptr : arena.push(g_all, s);
+  const int* did = id_all.empty() ? nullptr : arena.push(id_all, s);
   int p = 0;
   for (int p0 = 0; p0 < maxn; p0 += nb, p++) {
-    launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, stream_);
+    launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, s);
     const Step& st = steps[p];
-    if (st.ntr > 0) launch_lu_step(dtr + st.tr0, st.ntr, st.nwg, defer, stream_);
+    if (st.ntr > 0) launch_lu_step(dtr + st.tr0, st.ntr, st.nwg, defer, s);
     if (st.ng > 0)
       launch_gemm<double>(dg + st.g0, st.ng, st.ntiles, ws_.p, did ? did + st.id0 : nullptr,
-                          st.nid, st.bm, stream_, tune_.gemm_mfma != 0);
+                          st.nid, st.bm, s, tune_.gemm_mfma != 0);
   }
 }
 
@@ -1168,6 +1199,7 @@ void Engine::factorize_fp64(int which) {
   compress();
   Factorization& F = fac_[which];
   if (F.done) return;
+  finish_root();  // a previous factorization's root LU may still use F's buffers
   buf_gen_++;  // captured solves refer to the factorization's buffers
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
   factor_leaves();
@@ -1299,7 +1331,17 @@ void Engine::factorize_fp64(int which) {
     }
     run_gemm(gz);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
-    lu_batch(lu);
+    if (offd == 0 && overlap_root_) {
+      // the roots: nothing later in the factorization reads their LU, so it
+      // runs on aux_ and the solve that follows waits for it at its last level
+      HODLR_HIP_CHECK(hipEventRecord(ev_cap_, stream_));
+      HODLR_HIP_CHECK(hipStreamWaitEvent(aux_, ev_cap_, 0));
+      lu_batch(lu, aux_, aux_arena_);
+      HODLR_HIP_CHECK(hipEventRecord(ev_root_, aux_));
+      F.root_pending = true;
+    } else {
+      lu_batch(lu);
+    }
     if (offd > 0) {
       lu_solve_batch(sd);
       run_gemm(gu);
@@ -1357,6 +1399,7 @@ void Engine::set_graphs(bool on) {
  * is captured, only kernel launches).  Later calls copy X into the graph's
  * buffer, launch the graph and copy the result back. */
 void Engine::solve_fp64_graph(int which, double* X, int nrhs) {
+  finish_root();  // (captured and replayed work stays on stream_)
   const size_t N = (size_t)ntot_;
   grow(gx_, N * nrhs);
   const size_t bytes = N * nrhs * sizeof(double);
@@ -1481,6 +1524,7 @@ void Engine::solve_fp64_inplace(int which, double* X, int nrhs) {
     }
     run_gemm(gz);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
+    if (offd == 0) wait_root(F);
     lu_solve_batch(sd);
     run_gemm(gu);
   }
@@ -2092,6 +2136,7 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
 void Engine::logdet_terms(std::vector<double>& hl, std::vector<double>& hp, std::vector<int>& inst) {
   factorize_fp64(0);
   Factorization& F = fac_[0];
+  wait_root(F);
   std::vector<LuDesc> ld;
   inst.clear();
   for (auto& lf : leaves_) {

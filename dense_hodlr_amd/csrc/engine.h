@@ -166,6 +166,7 @@ struct MixedFactorization {
 
 struct Factorization {
   bool done = false;
+  bool root_pending = false;  // the roots' capacitance LU may still run on aux_
   DBuf<double> W;
   DBuf<double> C;
   DBuf<int> piv;
@@ -251,6 +252,11 @@ class Engine {
   void ensure_ws(size_t doubles);
   void grow(DBuf<double>& b, size_t doubles);
   void lu_batch(std::vector<LuDesc>& descs);
+  void lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& arena);
+  // the roots' LU, queued on aux_ by the factorization: stream_ waits for it
+  // (wait_root), or the host does and the aux arena is released (finish_root)
+  void wait_root(Factorization& F);
+  void finish_root();
   void factorize_fp64(int which);
   void logdet_terms(std::vector<double>& hl, std::vector<double>& hp, std::vector<int>& inst);
   void factorize_mixed(int which);
@@ -298,6 +304,13 @@ class Engine {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
   DescArena arena_;
+  // The roots' capacitance LU (one system per instance, the largest of the
+  // tree, a long chain of small launches) runs on aux_ while the solve that
+  // follows the factorization works through the lower levels on stream_.
+  hipStream_t aux_ = nullptr;
+  hipEvent_t ev_cap_ = nullptr, ev_root_ = nullptr;
+  DescArena aux_arena_;
+  bool overlap_root_ = true;
   DBuf<EntrySource> d_srcs_;
   std::vector<DBuf<double>> d_coords_;
   std::vector<DBuf<double>> d_dense_;

@@ -125,6 +125,45 @@ def test_exact_hodlr_dense_constructor(oracle):
     assert np.array_equal(g.nodes(), o.nodes())
 
 
+def test_root_lu_side_stream_matches_in_order(monkeypatch):
+    """The roots' capacitance LU runs on a second stream while the following
+    solve works through the lower levels (the solve waits for it at the
+    root level); switched off (HODLR_NO_ROOT_OVERLAP) everything runs in
+    order on one stream.  Same kernels, same data: recSM and recLU
+    solutions, the refined solve and the log-det are bit-identical, also for
+    a batch of instances (one root each) and across repeated solves."""
+    from dense_hodlr_amd import HODLR_Batch
+    _case("root LU side stream vs in order")
+    n = 6000
+    pts = I.points_2d_kd(n, 51)
+    b = I.rhs_normal(n, 4, 52)
+    bpts = [I.points_2d_kd(m, 60 + m) for m in (1500, 2600, 900)]
+    out = []
+    for flag in (None, "1"):
+        if flag:
+            monkeypatch.setenv("HODLR_NO_ROOT_OVERLAP", flag)
+        else:
+            monkeypatch.delenv("HODLR_NO_ROOT_OVERLAP", raising=False)
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=128)
+        g.set_LRTolerance(1e-10)
+        xs1 = g.recSM_Solve(b)
+        xs2 = g.recSM_Solve(b)
+        xl = g.recLU_Solve(b)
+        ld = g.logAbsDeterminant()
+        g.set_precision("fp64_refined")
+        xr = g.recSM_Solve(b)
+        del g
+        bt = HODLR_Batch(bpts, "se", 0.1, 1e-2, sizeThreshold=64)
+        bb = I.rhs_normal(sum(len(p) for p in bpts), 1, 53)
+        xb = bt.recSM_Solve(bb)
+        lb = bt.logAbsDeterminants()
+        del bt
+        out.append((xs1, xs2, xl, ld, xr, xb, lb))
+    for u, v in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(u), np.asarray(v))
+    assert np.array_equal(out[0][0], out[0][1])
+
+
 @pytest.mark.parametrize("n", [3000, 4500])
 def test_single_large_leaf_lu_paths(n):
     """One dense leaf of n rows (sizeThreshold above n): the batched LU at
