@@ -152,13 +152,14 @@ void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
  * units, lu_solve_small_wgs(ncols, maxn, wave) per problem (maxn: the
  * largest system of the launch; nwg: the units of all problems).  A unit is
  * a workgroup, or with wave set and maxn <= LU_SOLVE_WAVE_MAX a single wave
- * (the barrier-free kernel; same results).  Returns false, and launches
- * nothing, when maxn exceeds the limit. */
+ * (the barrier-free kernel; same results).  With blocked set, systems above
+ * 512 rows are substituted in blocks of 64 rows (same results).  Returns
+ * false, and launches nothing, when maxn exceeds the limit. */
 constexpr int LU_SOLVE_SMALL_MAX = 2048;
 constexpr int LU_SOLVE_WAVE_MAX = 512;
 int lu_solve_small_wgs(int ncols, int maxn, bool wave);
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
-                           hipStream_t s);
+                           bool blocked, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
                      hipStream_t s);
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);

@@ -1013,6 +1013,164 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
   }
 }
 
+/* The same solves for systems of 512-2048 rows, in blocks of 64 rows: the
+ * wave that holds a block's rows solves the block's diagonal part alone
+ * (staged in LDS, the step's value broadcast by readlane, no barrier), then
+ * the whole workgroup subtracts the block's 64 solved values from the rows
+ * below it (forward) or above it (backward): two barriers per block instead
+ * of one per row.  Every row still receives its updates one at a time, in
+ * the order of the step-by-step substitution, so results are bit-identical
+ * to lu_solve_small_kernel. */
+template <int R>
+__global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __restrict__ d,
+                                                           int count) {
+  constexpr int CB = solve_cols(R);
+  __shared__ int perm[R * TPB];
+  __shared__ int pv[R * TPB];
+  __shared__ double blk[64][65];  // diagonal block: blk[i][j] = A(b0 + i, b0 + j)
+  __shared__ double xs[64][CB];   // the block's solved values
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SolveDesc p = d[lo];
+  const int n = p.n, t = threadIdx.x;
+  const int c0 = ((int)blockIdx.x - p.chunk0) * CB;
+  const int nc = min(CB, p.ncols - c0);
+  // the interchanges piv[0..n), applied in order, as one permutation (read
+  // by all threads, replayed from LDS by one)
+  for (int i = t; i < n; i += TPB) {
+    perm[i] = i;
+    pv[i] = p.piv[i];
+  }
+  __syncthreads();
+  if (t == 0)
+    for (int k = 0; k < n; k++) {
+      const int q = pv[k];
+      if (q != k) {
+        const int x = perm[k];
+        perm[k] = perm[q];
+        perm[q] = x;
+      }
+    }
+  __syncthreads();
+  double x[R][CB];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+#pragma unroll
+    for (int c = 0; c < CB; c++)
+      x[r][c] = (i < n && c < nc) ? p.B[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
+  }
+  const double* LU = p.LU;
+  const size_t ld = (size_t)p.ld_lu;
+  const int lane = t & 63, wave = t >> 6;
+  const int nblk = (n + 63) / 64;
+  auto stage = [&](int b0, int bw) {
+    for (int e = t; e < 64 * 64; e += TPB) {
+      const int i = e & 63, j = e >> 6;
+      blk[i][j] = (i < bw && j < bw) ? LU[(size_t)(b0 + i) + (size_t)(b0 + j) * ld] : 0.0;
+    }
+  };
+  // forward: unit lower triangle
+  for (int bi = 0; bi < nblk; bi++) {
+    const int b0 = bi * 64, bw = min(64, n - b0);
+    stage(b0, bw);
+    __syncthreads();
+    if (wave == (bi & 3)) {  // rows b0 + lane, register slot b0 / 256
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        if (q == (b0 >> 8)) {
+          for (int l = 0; l < bw; l++) {
+            double xl[CB];
+#pragma unroll
+            for (int c = 0; c < CB; c++) xl[c] = wave_bcast(x[q][c], l);
+            if (lane > l && lane < bw) {
+              const double a = blk[lane][l];
+#pragma unroll
+              for (int c = 0; c < CB; c++) x[q][c] -= a * xl[c];
+            }
+          }
+          if (lane < bw)
+#pragma unroll
+            for (int c = 0; c < CB; c++) xs[lane][c] = x[q][c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int i = t + TPB * q;
+      if (i >= b0 + bw && i < n) {
+        const double* col = LU + (size_t)i + (size_t)b0 * ld;
+#pragma unroll 8
+        for (int j = 0; j < bw; j++) {
+          const double a = col[(size_t)j * ld];
+#pragma unroll
+          for (int c = 0; c < CB; c++) x[q][c] -= a * xs[j][c];
+        }
+      }
+    }
+    __syncthreads();  // blk and xs are rewritten for the next block
+  }
+  // backward: upper triangle with its diagonal
+  for (int bi = nblk - 1; bi >= 0; bi--) {
+    const int b0 = bi * 64, bw = min(64, n - b0);
+    stage(b0, bw);
+    __syncthreads();
+    if (wave == (bi & 3)) {
+#pragma unroll
+      for (int q = 0; q < R; q++)
+        if (q == (b0 >> 8)) {
+          for (int l = bw - 1; l >= 0; l--) {
+            if (lane == l) {
+              const double dg = blk[l][l];
+#pragma unroll
+              for (int c = 0; c < CB; c++) x[q][c] = x[q][c] / dg;
+            }
+            double xl[CB];
+#pragma unroll
+            for (int c = 0; c < CB; c++) xl[c] = wave_bcast(x[q][c], l);
+            if (lane < l) {
+              const double a = blk[lane][l];
+#pragma unroll
+              for (int c = 0; c < CB; c++) x[q][c] -= a * xl[c];
+            }
+          }
+          if (lane < bw)
+#pragma unroll
+            for (int c = 0; c < CB; c++) xs[lane][c] = x[q][c];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+      const int i = t + TPB * q;
+      if (i < b0) {
+        const double* col = LU + (size_t)i + (size_t)b0 * ld;
+#pragma unroll 8
+        for (int j = bw - 1; j >= 0; j--) {
+          const double a = col[(size_t)j * ld];
+#pragma unroll
+          for (int c = 0; c < CB; c++) x[q][c] -= a * xs[j][c];
+        }
+      }
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = t + TPB * r;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < CB; c++)
+        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+  }
+}
+
 constexpr int wave_rows(int maxn) { return maxn <= 64 ? 1 : maxn <= 128 ? 2 : maxn <= 256 ? 4 : 8; }
 // columns per wave: fewer for more rows per lane (registers: R x CB values
 // and the D x R ring; beyond 8 rows per lane the compiler no longer keeps
@@ -1083,7 +1241,7 @@ int lu_solve_small_wgs(int ncols, int maxn, bool wave) {
 }
 
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
-                           hipStream_t s) {
+                           bool blocked, hipStream_t s) {
   if (maxn > LU_SOLVE_SMALL_MAX) return false;
   if (count == 0 || nwg == 0) return true;
   if (wave && maxn <= LU_SOLVE_WAVE_MAX) {
@@ -1094,6 +1252,14 @@ bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, boo
       case 4: lu_solve_wave_kernel<4, 8, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
       default: lu_solve_wave_kernel<8, 4, 4><<<grid, TPB, 0, s>>>(d, count, nwg); break;
     }
+    HODLR_HIP_CHECK(hipGetLastError());
+    return true;
+  }
+  if (blocked && maxn > 512) {
+    if (solve_rows(maxn) == 4)
+      lu_solve_blk_kernel<4><<<nwg, TPB, 0, s>>>(d, count);
+    else
+      lu_solve_blk_kernel<8><<<nwg, TPB, 0, s>>>(d, count);
     HODLR_HIP_CHECK(hipGetLastError());
     return true;
   }

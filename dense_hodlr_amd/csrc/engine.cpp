@@ -75,6 +75,8 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   // and the one-wave-per-system kernel (0: the workgroup kernel for all)
   if (const char* e = std::getenv("HODLR_LU_SOLVE_WAVE_MAX"))
     solve_wave_max_ = std::max(0, std::min(LU_SOLVE_WAVE_MAX, std::atoi(e)));
+  // and the blocked one-launch solve above 512 rows (off: step by step)
+  solve_blk_ = std::getenv("HODLR_NO_LU_SOLVE_BLOCKS") == nullptr;
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreate(&ev0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
@@ -543,7 +545,7 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
       nwg += lu_solve_small_wgs(d.ncols, maxn, wave);
     }
     launch_lu_solve_small(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, wave,
-                          stream_);
+                          solve_blk_, stream_);
     return;
   }
   {

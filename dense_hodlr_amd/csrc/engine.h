@@ -299,6 +299,7 @@ class Engine {
   bool trace_ = false;
   int solve_small_max_ = LU_SOLVE_SMALL_MAX;  // one-launch LU solves up to this size
   int solve_wave_max_ = LU_SOLVE_WAVE_MAX;    // of which one wave per system up to this size
+  bool solve_blk_ = true;                     // and in blocks of 64 rows above 512
   std::vector<hipEvent_t> prof_ev_;
 
   hipStream_t stream_ = nullptr;

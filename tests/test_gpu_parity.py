@@ -125,6 +125,42 @@ def test_exact_hodlr_dense_constructor(oracle):
     assert np.array_equal(g.nodes(), o.nodes())
 
 
+@pytest.mark.parametrize("n", [900, 1500])
+def test_blocked_one_launch_solve_matches_step_kernel(monkeypatch, n):
+    """Systems of 513-2048 rows are substituted in blocks of 64 rows (the
+    owning wave solves the diagonal block, the workgroup updates the rest);
+    each row receives its updates in the same order as in the step-by-step
+    kernel, so the results are bit-identical with the blocks switched off
+    (HODLR_NO_LU_SOLVE_BLOCKS).  One dense leaf of n rows (the leaf inverse
+    solves n columns through it), random (pivoting at most columns), and a
+    kernel matrix whose leaves have ~750 rows."""
+    _case(f"blocked one-launch solve n={n}")
+    rng = np.random.default_rng(7 + n)
+    A = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, 3))
+    pts = I.points_2d_kd(6000, 71)
+    bk = I.rhs_normal(6000, 2, 72)
+    out = []
+    for flag in (None, "1"):
+        if flag:
+            monkeypatch.setenv("HODLR_NO_LU_SOLVE_BLOCKS", flag)
+        else:
+            monkeypatch.delenv("HODLR_NO_LU_SOLVE_BLOCKS", raising=False)
+        g = HODLR_Matrix(A, n + 1, "partialPiv_ACA")
+        xs = g.recSM_Solve(b)
+        ld = g.logAbsDeterminant()
+        del g
+        h = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-3, sizeThreshold=800)
+        h.set_LRTolerance(1e-10)
+        xk = h.recSM_Solve(bk)
+        xl = h.recLU_Solve(bk)
+        del h
+        out.append((xs, ld, xk, xl))
+    for u, v in zip(out[0], out[1]):
+        assert np.array_equal(np.asarray(u), np.asarray(v))
+    assert rel(out[0][0], np.linalg.solve(A, b)) < 1e-13 * np.linalg.cond(A)
+
+
 def test_root_lu_side_stream_matches_in_order(monkeypatch):
     """The roots' capacitance LU runs on a second stream while the following
     solve works through the lower levels (the solve waits for it at the
