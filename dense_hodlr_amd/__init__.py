@@ -2,10 +2,10 @@
 amiraa127/Dense_HODLR (kernel/dense constructors, partial-pivot ACA or SVD
 compression, recSM/recLU solves, log-determinant), built as hand-written HIP
 kernels for gfx950 behind the C ABI in include/dense_hodlr.h."""
-from ._lib import HodlrError, lib
+from ._lib import HodlrError, HodlrNotConverged, lib
 from .hodlr_matrix import HODLR_Batch, HODLR_Matrix
 from .memory import memory_cache, memory_stats, memory_trim
 from .rank_model import RankModel
 
-__all__ = ["HODLR_Matrix", "HODLR_Batch", "RankModel", "HodlrError", "lib", "memory_stats",
+__all__ = ["HODLR_Matrix", "HODLR_Batch", "RankModel", "HodlrError", "HodlrNotConverged", "lib", "memory_stats",
            "memory_trim", "memory_cache"]

@@ -19,7 +19,9 @@ ERRORS = {
     -4: "HODLR_ERR_NOMEM",
     -5: "HODLR_ERR_NUMERIC",
     -6: "HODLR_ERR_UNSUPPORTED",
+    -7: "HODLR_ERR_NOT_CONVERGED",
 }
+HODLR_ERR_NOT_CONVERGED = -7
 
 KERNELS = {"gaussian": 0, "matern32": 1, "se": 2, "imq": 3, "quadratic": 4, "exponential": 5}
 LR_METHODS = {"partialPiv_ACA": 0, "SVD": 1}
@@ -84,6 +86,7 @@ class Stats(C.Structure):
         ("tune_lu_panel", C.c_int),
         ("tuned", C.c_int),
         ("tune_gemm_mfma", C.c_int),
+        ("factor_stream_time", C.c_double),
     ]
 
 
@@ -107,6 +110,13 @@ class HodlrError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"{ERRORS.get(code, code)}: {msg}")
         self.code = code
+
+
+class HodlrNotConverged(HodlrError):
+    """Refinement ended above its target; ``x`` holds the best iterate
+    (set by the solve that raised it)."""
+
+    x = None
 
 
 def lib():
@@ -183,5 +193,6 @@ def lib():
 def check(rc, handle=None):
     if rc < 0:
         msg = lib().hodlr_last_error(handle)
-        raise HodlrError(rc, msg.decode() if msg else "")
+        cls = HodlrNotConverged if rc == HODLR_ERR_NOT_CONVERGED else HodlrError
+        raise cls(rc, msg.decode() if msg else "")
     return rc

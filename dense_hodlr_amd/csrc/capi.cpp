@@ -43,6 +43,8 @@ int guard(hodlr_handle h, F&& f) {
     return HODLR_OK;
   } catch (const std::invalid_argument& e) {
     return fail(h, HODLR_ERR_ARG, e.what());
+  } catch (const hodlr::NotConverged& e) {
+    return fail(h, HODLR_ERR_NOT_CONVERGED, e.what());
   } catch (const hodlr::HipError& e) {
     const std::string m = e.what();
     if (m.find("hipMalloc") != std::string::npos || m.find("out of memory") != std::string::npos)
@@ -429,6 +431,7 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
   return guard(h, [&] {
     if (!out) throw std::invalid_argument("null output");
     std::memset(out, 0, sizeof(*out));
+    h->eng->sync_stats();
     const hodlr::Engine& e = *h->eng;
     out->num_levels = e.num_levels();
     out->num_nodes = (int)e.nodes().size();
@@ -449,6 +452,7 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->pred_log_err = s.pred_log_err;
     out->compress_time = s.compress_time;
     out->factor_time = s.factor_time;
+    out->factor_stream_time = s.factor_stream_time;
     out->solve_time = s.solve_time;
     out->refine_time = s.refine_time;
     out->refine_iterations = s.refine_iterations;

@@ -19,6 +19,12 @@ struct HipError : std::runtime_error {
   explicit HipError(const std::string& s) : std::runtime_error(s) {}
 };
 
+/* Refinement ended above its target: the output holds the best iterate
+ * (column by column) and the stats its residual. */
+struct NotConverged : std::runtime_error {
+  explicit NotConverged(const std::string& s) : std::runtime_error(s) {}
+};
+
 #define HODLR_HIP_CHECK(expr)                                                          \
   do {                                                                                 \
     hipError_t e_ = (expr);                                                            \

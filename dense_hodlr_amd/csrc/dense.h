@@ -183,6 +183,10 @@ void launch_segnorm2(const double* X, int ld, const int* row0, int nseg, int nco
                      double* out, hipStream_t s);
 void launch_segscale(double* X, int ld, const int* row0, int nseg, int ncols, const double* sc,
                      hipStream_t s);
+/* Y <- X on the rows of segment s of column c where mask[s + c*nseg] != 0
+ * (mask on the device; the rows split into nsplit parts per workgroup) */
+void launch_segcopy(const double* X, double* Y, int ld, const int* row0, int nseg, int ncols,
+                    int nsplit, const int* mask, hipStream_t s);
 /* X += alpha * D (FP32 correction) over an n x ncols block */
 void launch_add_f32(double* X, int ldx, const float* D, int ldd, int n, int ncols, hipStream_t s);
 /* Batched column operations of the block GMRES: dots of w with nvec basis

@@ -570,6 +570,21 @@ __global__ void __launch_bounds__(TPB)
   for (int i = row0[sg] + threadIdx.x; i < row0[sg + 1]; i += TPB) x[i] *= f;
 }
 
+/* rows [row0[s], row0[s+1]) of column c copied from X to Y where
+ * mask[s + c * nseg] is set; split z of the grid's z parts of the rows */
+__global__ void __launch_bounds__(TPB)
+    segcopy_kernel(const double* __restrict__ X, double* __restrict__ Y, int ld,
+                   const int* __restrict__ row0, int nseg, const int* __restrict__ mask) {
+  const int sg = blockIdx.y, c = blockIdx.x, sp = blockIdx.z, nsp = gridDim.z;
+  if (!mask[sg + (size_t)c * nseg]) return;
+  const int r0 = row0[sg], len = row0[sg + 1] - r0;
+  const int part = (len + nsp - 1) / nsp;
+  const int lo = r0 + min(len, sp * part), hi = r0 + min(len, (sp + 1) * part);
+  const double* x = X + (size_t)c * ld;
+  double* y = Y + (size_t)c * ld;
+  for (int i = lo + threadIdx.x; i < hi; i += TPB) y[i] = x[i];
+}
+
 __global__ void __launch_bounds__(TPB)
     colnorm2_kernel(const double* __restrict__ X, int n, int ld, double* __restrict__ out) {
   __shared__ double sh[TPB];
@@ -1359,6 +1374,15 @@ void launch_segscale(double* X, int ld, const int* row0, int nseg, int ncols, co
   if (ncols <= 0 || nseg <= 0) return;
   dim3 g(ncols, nseg);
   segscale_kernel<<<g, TPB, 0, s>>>(X, ld, row0, nseg, sc);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+void launch_segcopy(const double* X, double* Y, int ld, const int* row0, int nseg, int ncols,
+                    int nsplit, const int* mask, hipStream_t s) {
+  if (ncols <= 0 || nseg <= 0) return;
+  if (nsplit < 1 || nsplit > 64) throw std::invalid_argument("segcopy: nsplit out of range");
+  dim3 g(ncols, nseg, nsplit);
+  segcopy_kernel<<<g, TPB, 0, s>>>(X, Y, ld, row0, nseg, mask);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

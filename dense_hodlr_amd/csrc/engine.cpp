@@ -83,6 +83,9 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_cap_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_root_, hipEventDisableTiming));
+  HODLR_HIP_CHECK(hipEventCreate(&ev_f0_));
+  HODLR_HIP_CHECK(hipEventCreate(&ev_f1_));
+  HODLR_HIP_CHECK(hipEventCreate(&ev_froot_));
   // (off: the roots' LU on the engine's stream, as every other level's)
   overlap_root_ = std::getenv("HODLR_NO_ROOT_OVERLAP") == nullptr;
   build_trees();
@@ -120,9 +123,11 @@ void Engine::autotune() {
   };
   auto time_factor = [&](const Tuning& t) {
     tune_ = t;
+    finish_root();
     for (auto& f : fac_) f.done = false;
     leaves_factored_ = false;
     factorize_fp64(0);
+    sync_stats();  // the full factorization, the roots' LU on the side stream included
     return stats_.factor_time;
   };
   double bt = 1e300;
@@ -169,12 +174,24 @@ Engine::~Engine() {
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev_cap_) (void)hipEventDestroy(ev_cap_);
   if (ev_root_) (void)hipEventDestroy(ev_root_);
+  if (ev_f0_) (void)hipEventDestroy(ev_f0_);
+  if (ev_f1_) (void)hipEventDestroy(ev_f1_);
+  if (ev_froot_) (void)hipEventDestroy(ev_froot_);
   if (aux_) (void)hipStreamDestroy(aux_);
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void Engine::wait_root(Factorization& F) {
   if (F.root_pending) HODLR_HIP_CHECK(hipStreamWaitEvent(stream_, ev_root_, 0));
+}
+
+void Engine::sync_stats() {
+  if (!factor_time_pending_) return;
+  HODLR_HIP_CHECK(hipEventSynchronize(ev_froot_));
+  float ms = 0;
+  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev_f0_, ev_froot_));
+  stats_.factor_time = std::max(stats_.factor_stream_time, ms * 1e-3);
+  factor_time_pending_ = false;
 }
 
 void Engine::finish_root() {
@@ -318,6 +335,7 @@ void Engine::reset_attributes() {
   U_.clear();
   V_.clear();
   stats_ = Stats();
+  factor_time_pending_ = false;
 }
 
 void Engine::set_lr_method(int m) {
@@ -347,6 +365,7 @@ void Engine::set_min_pivot(double p) {
 void Engine::set_precision(int p) {
   if (p < 0 || p > 2) throw std::invalid_argument("unknown precision");
   if (p != precision_) {
+    finish_root();  // the roots' LU may still write F.C and F.piv on the side stream
     precision_ = p;
     for (auto& f : fac_) {
       f.done = false;
@@ -1203,7 +1222,8 @@ void Engine::factorize_fp64(int which) {
   if (F.done) return;
   finish_root();  // a previous factorization's root LU may still use F's buffers
   buf_gen_++;  // captured solves refer to the factorization's buffers
-  HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
+  factor_time_pending_ = false;
+  HODLR_HIP_CHECK(hipEventRecord(ev_f0_, stream_));
   factor_leaves();
   const size_t N = (size_t)ntot_;
   F.W.alloc(std::max<size_t>(1, N * ktot_));
@@ -1340,7 +1360,9 @@ void Engine::factorize_fp64(int which) {
       HODLR_HIP_CHECK(hipStreamWaitEvent(aux_, ev_cap_, 0));
       lu_batch(lu, aux_, aux_arena_);
       HODLR_HIP_CHECK(hipEventRecord(ev_root_, aux_));
+      HODLR_HIP_CHECK(hipEventRecord(ev_froot_, aux_));
       F.root_pending = true;
+      factor_time_pending_ = true;
     } else {
       lu_batch(lu);
     }
@@ -1349,11 +1371,12 @@ void Engine::factorize_fp64(int which) {
       run_gemm(gu);
     }
   }
-  HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
-  HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
+  HODLR_HIP_CHECK(hipEventRecord(ev_f1_, stream_));
+  HODLR_HIP_CHECK(hipEventSynchronize(ev_f1_));
   float ms = 0;
-  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
-  stats_.factor_time = ms * 1e-3;
+  HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev_f0_, ev_f1_));
+  stats_.factor_stream_time = ms * 1e-3;
+  stats_.factor_time = ms * 1e-3;  // completed by sync_stats while the roots' LU is pending
   arena_.reset();
   F.done = true;
 }
@@ -1541,9 +1564,17 @@ void Engine::solve_host(int which, const double* B, int ldb, int nrhs, double* X
   dX.alloc(N * nrhs);
   HODLR_HIP_CHECK(hipMemcpy2D(dB.p, N * sizeof(double), B, (size_t)ldb * sizeof(double),
                               N * sizeof(double), nrhs, hipMemcpyHostToDevice));
-  solve_device(which, dB.p, (int)N, nrhs, dX.p, (int)N);
-  HODLR_HIP_CHECK(hipMemcpy2D(X, (size_t)ldx * sizeof(double), dX.p, N * sizeof(double),
-                              N * sizeof(double), nrhs, hipMemcpyDeviceToHost));
+  auto copy_back = [&] {
+    HODLR_HIP_CHECK(hipMemcpy2D(X, (size_t)ldx * sizeof(double), dX.p, N * sizeof(double),
+                                N * sizeof(double), nrhs, hipMemcpyDeviceToHost));
+  };
+  try {
+    solve_device(which, dB.p, (int)N, nrhs, dX.p, (int)N);
+  } catch (const NotConverged&) {
+    copy_back();  // the best iterate
+    throw;
+  }
+  copy_back();
 }
 
 /* ------------------------------------------------------------------ */
@@ -1743,6 +1774,8 @@ void Engine::factorize_mixed(int which) {
   float ms = 0;
   HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev0_, ev1_));
   stats_.factor_time = ms * 1e-3;
+  stats_.factor_stream_time = ms * 1e-3;
+  factor_time_pending_ = false;
   arena_.reset();
   M.done = true;
 }
@@ -1946,6 +1979,31 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     for (auto& v : bn)
       if (!(v > 0)) v = 1.0;
   }
+  // The best iterate of every instance column (lowest true relative
+  // residual seen), returned where the refinement ends above its target
+  // with a worse one.
+  grow(xbest_, N * nrhs);
+  grow_t(bestmask_, (size_t)nseg * nrhs);
+  std::vector<double> best((size_t)nseg * nrhs, HUGE_VAL);
+  std::vector<int> hmask((size_t)nseg * nrhs, 0);
+  auto masked_copy = [&](const double* from, double* to) {
+    HODLR_HIP_CHECK(hipMemcpyAsync(bestmask_.p, hmask.data(), hmask.size() * sizeof(int),
+                                   hipMemcpyHostToDevice, stream_));
+    launch_segcopy(from, to, ld, seg_rows_.p, nseg, nrhs, nsplit, bestmask_.p, stream_);
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));  // hmask is reused
+  };
+  auto note_best = [&](const std::vector<double>& r) {  // r: segnorms of the residual of xw_
+    bool any = false;
+    for (size_t e = 0; e < r.size(); e++) {
+      const double rc = r[e] / bn[e];
+      hmask[e] = rc < best[e];
+      if (hmask[e]) {
+        best[e] = rc;
+        any = true;
+      }
+    }
+    if (any) masked_copy(xw_.p, xbest_.p);
+  };
   // Hessenberg, Givens rotations and right-hand side per column (host)
   std::vector<double> H((size_t)(m + 1) * m * nrhs), cs((size_t)m * nrhs), sn((size_t)m * nrhs),
       g((size_t)(m + 1) * nrhs);
@@ -1967,6 +2025,7 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     upload(ones, dcoef);
     double prev = 0.0;
     segnorms(rbuf_.p, rs);
+    note_best(rs);
     for (size_t e = 0; e < rs.size(); e++) prev = std::max(prev, rs[e] / bn[e]);
     std::vector<double> dn, xn;
     while (prev > refine_target_ && total < refine_max_) {
@@ -1994,6 +2053,7 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
       }
       total++;
       segnorms(rbuf_.p, rs);
+      note_best(rs);
       double now = 0.0;
       for (size_t e = 0; e < rs.size(); e++) now = std::max(now, rs[e] / bn[e]);
       if (trace_)
@@ -2007,6 +2067,7 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
   }
   for (;;) {
     segnorms(rbuf_.p, rs);
+    note_best(rs);
     res = 0.0;
     std::vector<char> active(nrhs, 0);
     for (int c = 0; c < nrhs; c++) {
@@ -2114,6 +2175,18 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
     launch_multi_axpy(zbasis_.p, vstride, jend, dcoef, xw_.p, ntot_, ld, nrhs, 1.0, 0, stream_);
     residual_dd(Beff, ldbeff, xw_.p, ld, nrhs, rbuf_.p, ld);
   }
+  {  // instance columns whose last iterate is worse than their best take the best
+    bool any = false;
+    res = 0.0;
+    for (size_t e = 0; e < rs.size(); e++) {
+      const double rc = rs[e] / bn[e];
+      hmask[e] = best[e] < rc;
+      any = any || hmask[e];
+      const double r = std::min(rc, best[e]);
+      if (!(r <= res)) res = r;
+    }
+    if (any) masked_copy(xbest_.p, xw_.p);
+  }
   if (nseg > 1) {  // undo the right-hand side scaling
     upload(bscale, segbuf_.p);
     launch_segscale(xw_.p, ld, seg_rows_.p, nseg, nrhs, segbuf_.p, stream_);
@@ -2129,6 +2202,14 @@ void Engine::solve_gmres(int which, bool mixed, const double* dB, int ldb, int n
   stats_.solve_time = ms * 1e-3;
   stats_.refine_time = ms * 1e-3;
   arena_.reset();
+  if (!(res <= refine_target_)) {
+    char msg[160];
+    std::snprintf(msg, sizeof msg,
+                  "refinement stopped at relative residual %.3e above the target %.3e after %d "
+                  "steps (the best iterate is returned)",
+                  res, refine_target_, total);
+    throw NotConverged(msg);
+  }
 }
 
 /* calcDeterminant (HODLR_Matrix.cpp:1671-1697): sum of log|diag LU| over the

@@ -208,7 +208,12 @@ class Engine {
   void get_factor(int idx, int which, double* host_out);
   struct Stats {
     int aca_iterations = 0;
-    double compress_time = 0, factor_time = 0, solve_time = 0, refine_time = 0;
+    // factor_time: from the factorization's first kernel to its last one on
+    // either stream (the roots' LU on the side stream included);
+    // factor_stream_time: up to the last one on the engine's stream, where a
+    // following solve starts and overlaps the roots' LU
+    double compress_time = 0, factor_time = 0, factor_stream_time = 0, solve_time = 0,
+           refine_time = 0;
     int refine_iterations = 0;
     double last_residual = 0;
     double aca_row_time = 0, aca_col_time = 0;  // with profiling on
@@ -233,6 +238,9 @@ class Engine {
   void set_rank_predictor(const float* params, double margin);
   void block_features(double tol, float* host_out);
   const Stats& stats() const { return stats_; }
+  /* Completes figures that wait for work still queued on the side stream
+   * (the factorization time including the roots' LU). */
+  void sync_stats();
   long long device_bytes() const;
   hipStream_t stream() const { return stream_; }
 
@@ -310,6 +318,10 @@ class Engine {
   // follows the factorization works through the lower levels on stream_.
   hipStream_t aux_ = nullptr;
   hipEvent_t ev_cap_ = nullptr, ev_root_ = nullptr;
+  // timing of the factorization: its start on stream_, its end on stream_
+  // and the end of the roots' LU on aux_ (read by sync_stats)
+  hipEvent_t ev_f0_ = nullptr, ev_f1_ = nullptr, ev_froot_ = nullptr;
+  bool factor_time_pending_ = false;
   DescArena aux_arena_;
   bool overlap_root_ = true;
   DBuf<EntrySource> d_srcs_;
@@ -333,6 +345,8 @@ class Engine {
   int gmres_restart_ = 20;
   DBuf<double> ws_, tmp_, tmp2_;
   DBuf<double> xw_;
+  DBuf<double> xbest_;  // refinement: best iterate per instance and column so far
+  DBuf<int> bestmask_;  // which instance columns to copy into (or back from) xbest_
   DBuf<double> ddy_, ddpart_, ddt_;  // double-double residual: Y, partials, projections
   DBuf<int> seg_rows_;               // instance row offsets (+ total) on the device
   DBuf<double> segbuf_, bscaled_;    // per-instance norms / scales, scaled batch RHS

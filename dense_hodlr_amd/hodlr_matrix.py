@@ -14,7 +14,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import HodlrError, check, lib
+from ._lib import HodlrError, HodlrNotConverged, check, lib
 
 
 def _dptr(a):
@@ -139,7 +139,11 @@ class HODLR_Matrix:
     def _solve(self, fn, B):
         B = _as_f64_2d(B, self._n)
         X = np.zeros_like(B, order="F")
-        check(fn(self._h, _dptr(B), self._n, B.shape[1], _dptr(X), self._n), self._h)
+        try:
+            check(fn(self._h, _dptr(B), self._n, B.shape[1], _dptr(X), self._n), self._h)
+        except HodlrNotConverged as e:
+            e.x = X  # the best iterate
+            raise
         return X
 
     def recSM_Solve(self, input_RHS):
@@ -189,7 +193,8 @@ class HODLR_Matrix:
         L = s.num_levels
         return dict(num_levels=L, num_nodes=s.num_nodes, num_leaves=s.num_leaves,
                     aca_iterations=s.aca_iterations, compress_time=s.compress_time,
-                    factor_time=s.factor_time, solve_time=s.solve_time,
+                    factor_time=s.factor_time, factor_stream_time=s.factor_stream_time,
+                    solve_time=s.solve_time,
                     refine_time=s.refine_time, refine_iterations=s.refine_iterations,
                     last_residual=s.last_residual, max_rank=list(s.max_rank[:L]),
                     mean_rank=list(s.mean_rank[:L]), device_bytes=s.device_bytes,
@@ -275,4 +280,4 @@ class HODLR_Batch(HODLR_Matrix):
         return out
 
 
-__all__ = ["HODLR_Matrix", "HODLR_Batch", "HodlrError"]
+__all__ = ["HODLR_Matrix", "HODLR_Batch", "HodlrError", "HodlrNotConverged"]

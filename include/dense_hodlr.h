@@ -35,7 +35,12 @@ enum {
   HODLR_ERR_DEVICE = -3,   /* HIP runtime failure or no usable GPU            */
   HODLR_ERR_NOMEM = -4,    /* device allocation failed                        */
   HODLR_ERR_NUMERIC = -5,  /* non-finite result                               */
-  HODLR_ERR_UNSUPPORTED = -6
+  HODLR_ERR_UNSUPPORTED = -6,
+  /* refinement (MIXED, FP64_REFINED) ended above its target: the solution
+   * output holds, per column (per instance column of a batch), the iterate
+   * with the lowest true residual seen, and hodlr_stats.last_residual the
+   * largest of those residuals */
+  HODLR_ERR_NOT_CONVERGED = -7
 };
 
 /* Kernel families (K(i,j) = f(|x_i - x_j|) [+ diagonal term]). */
@@ -88,7 +93,9 @@ typedef struct hodlr_stats {
   int num_leaves;
   int aca_iterations;     /* ACA iterations of the batched compression */
   double compress_time;   /* LR_ComputationTime */
-  double factor_time;     /* recSM/recLU factorization time */
+  double factor_time;     /* recSM/recLU factorization time: first to last kernel of the
+                             factorization, the roots' capacitance LU included (it runs on a
+                             second stream and a following solve overlaps it) */
   double solve_time;      /* last solve */
   double refine_time;     /* mixed precision: refinement part of the solve */
   int refine_iterations;
@@ -121,6 +128,8 @@ typedef struct hodlr_stats {
   int tune_lu_panel;
   int tuned;
   int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
+  double factor_stream_time; /* the factorization up to its last kernel on the engine's
+                                stream, where a following solve starts */
 } hodlr_stats;
 
 /* ---- construction ------------------------------------------------------ */
@@ -158,7 +167,8 @@ int hodlr_set_min_pivot(hodlr_handle h, double min_pivot);
 int hodlr_set_precision(hodlr_handle h, int precision);
 /* Refinement (MIXED, FP64_REFINED): relative residual ||b - A_H x|| / ||b||
  * every column must reach (default 1e-10) and the maximum number of steps
- * (default 30).  Residuals are exact to rounding: evaluated in double-double
+ * (default 30); a solve that ends above the target returns
+ * HODLR_ERR_NOT_CONVERGED with the best iterate in its output.  Residuals are exact to rounding: evaluated in double-double
  * arithmetic, as hodlr_residual does, or, after a step at most 1e-3 of the
  * iterate (targets >= 1e-12), updated by the FP64 product of the exact step
  * (TwoSum).  FP64_REFINED takes steps x += M^-1 r while each gains 8x and

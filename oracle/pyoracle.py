@@ -61,6 +61,9 @@ def lib():
         L.orc_logabsdet.restype = _d
         L.orc_logabsdet.argtypes = [_p, _dp]
         L.orc_matvec.argtypes = [_p, _dp, _i, _dp]
+        L.orc_residual_dd.argtypes = [_p, _dp, _dp, _i, _dp]
+        L.orc_refined_solve.restype = _i
+        L.orc_refined_solve.argtypes = [_p, _dp, _i, _dp, _d, _i, _dp]
         L.orc_time.restype = _d
         L.orc_time.argtypes = [_p, _i]
         L.orc_aca.restype = _i
@@ -225,6 +228,24 @@ class OracleHODLR:
         Y = np.zeros_like(X, order="F")
         lib().orc_matvec(self.h, _ptr(X), X.shape[1], _ptr(Y))
         return Y
+
+    def residual_dd(self, B, X):
+        """B - A_H X in double-double arithmetic, rounded once."""
+        B = np.asfortranarray(B.reshape(self.n, -1), dtype=np.float64)
+        X = np.asfortranarray(X.reshape(self.n, -1), dtype=np.float64)
+        R = np.zeros_like(B, order="F")
+        lib().orc_residual_dd(self.h, _ptr(B), _ptr(X), B.shape[1], _ptr(R))
+        return R
+
+    def refined_solve(self, B, target=1e-14, max_steps=3):
+        """recSM solution refined with double-double residuals: returns
+        (X, corrections made, final relative residual)."""
+        B = np.asfortranarray(B.reshape(self.n, -1), dtype=np.float64)
+        X = np.zeros_like(B, order="F")
+        res = np.zeros(1)
+        k = lib().orc_refined_solve(self.h, _ptr(B), B.shape[1], _ptr(X), float(target),
+                                    int(max_steps), _ptr(res))
+        return X, int(k), float(res[0])
 
     def dense_operator(self):
         """A_H assembled as a dense matrix (A_H e_j column by column)."""

@@ -1,0 +1,163 @@
+"""The large workloads at their full sizes on the GPU:
+
+* N = 262144 two-dimensional Gaussian system (l = 0.05, shift 1e-4, leaf
+  256, ACA tolerance 1e-10, 64 right-hand sides), refined to 1e-10 with and
+  without the learned rank predictor;
+* 512 batched Gaussian-process systems (N log-uniform in [1024, 16384]),
+  every instance refined to 1e-10, sampled instances against the oracle;
+* the predictor deployed on N = 262144 Matern-3/2 with l = 0.1.
+
+Each is a few seconds of GPU time; inputs are generated on the host."""
+import sys
+
+import numpy as np
+import pytest
+
+from dense_hodlr_amd import HODLR_Batch, HODLR_Matrix, RankModel
+from dense_hodlr_amd import inputs as I
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(msg):
+    print(f"[case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def colres(g, x, b):
+    r = g.residual(b, x)
+    return np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0)
+
+
+@pytest.fixture(scope="module")
+def gauss2d():
+    _case("inputs gauss2d_262144")
+    return I.config_inputs("gauss2d_262144")
+
+
+def _gauss2d_handle(cfg, precision="fp64_refined", model=None):
+    g = HODLR_Matrix.from_kernel(cfg["points"], "gaussian", 0.05, 1e-4, sizeThreshold=256)
+    g.set_LRTolerance(1e-10)
+    g.set_precision(precision)
+    g.set_refinement(1e-10, 30)
+    if model is not None:
+        g.set_rank_predictor(model, 0.25)
+    return g
+
+
+def test_gauss2d_262144_refined_learned_and_plain(gauss2d):
+    """Every column reaches 1e-10 against A_H by an independent
+    double-double evaluation of the returned solution; the residual the
+    refinement reports is that one; the predictor-sized run and the plain
+    run give identical ranks and bit-identical solutions (the predictor only
+    sizes storage)."""
+    cfg = gauss2d
+    b = cfg["rhs"]
+    _case("gauss2d_262144 fp64_refined + predictor")
+    g = _gauss2d_handle(cfg, model=RankModel.load())
+    x1 = g.recSM_Solve(b)
+    st = g.stats()
+    r1 = colres(g, x1, b)
+    n1 = g.nodes()
+    print(f"learned: reported {st['last_residual']:.3e}, checked max {r1.max():.3e}, "
+          f"steps {st['refine_iterations']}, misses {st['pred_misses']}/{st['pred_blocks']}",
+          flush=True)
+    assert r1.max() <= 1e-10
+    assert abs(st["last_residual"] - r1.max()) <= 1e-3 * r1.max()
+    assert st["pred_blocks"] == 2 * len(n1)
+    del g
+    _case("gauss2d_262144 fp64_refined plain")
+    h = _gauss2d_handle(cfg)
+    x2 = h.recSM_Solve(b)
+    assert np.array_equal(n1, h.nodes())
+    assert np.array_equal(x1, x2)
+
+
+@pytest.mark.skip(reason="the mixed-precision factorization does not converge at this condition "
+                         "number (~2e7: kappa times the FP32 unit roundoff exceeds 1); it stops "
+                         "with HODLR_ERR_NOT_CONVERGED near relative residual 1")
+def test_gauss2d_262144_mixed_reaches_target(gauss2d):
+    cfg = gauss2d
+    g = _gauss2d_handle(cfg, precision="mixed")
+    x = g.recSM_Solve(cfg["rhs"])
+    assert colres(g, x, cfg["rhs"]).max() <= 1e-10
+
+
+@pytest.fixture(scope="module")
+def gp512():
+    _case("inputs gp_batch_512")
+    return I.config_inputs("gp_batch_512")
+
+
+@pytest.mark.parametrize("prec", ["fp64_refined", "mixed"])
+def test_gp_batch_512_every_instance(gp512, oracle, prec):
+    """All 512 systems in one handle, each refined to 1e-10 on its own
+    (independent double-double check per instance), log-determinants
+    finite; eight sampled instances (N <= 4096) against the oracle's
+    refined solution (1e-10) and log-determinant (1e-12)."""
+    cfg = gp512
+    _case(f"gp_batch_512 {prec}")
+    g = HODLR_Batch(cfg["points"], "se", cfg["length_scale"], cfg["diag"],
+                    sizeThreshold=cfg["leaf"])
+    g.set_LRTolerance(cfg["tol"])
+    g.set_precision(prec)
+    g.set_refinement(1e-10, 40)
+    B = np.concatenate(cfg["rhs"])
+    X = g.recSM_Solve(B)
+    R = g.residual(B, X)
+    rel = np.array([np.linalg.norm(r) / np.linalg.norm(b) for r, b in zip(g.split(R), g.split(B))])
+    st = g.stats()
+    lds = g.logAbsDeterminants()
+    print(f"{prec}: {len(rel)} instances, {int(sum(cfg['n']))} rows, max instance residual "
+          f"{rel.max():.2e}, steps {st['refine_iterations']}", flush=True)
+    assert len(rel) == 512 and rel.max() <= 1e-10
+    assert np.all(np.isfinite(lds))
+    small = [q for q in range(512) if cfg["n"][q] <= 4096]
+    sample = small[:: max(1, len(small) // 8)][:8]
+    Xs = g.split(X)
+    for q in sample:
+        o = oracle.OracleHODLR(cfg["n"][q], leaf=cfg["leaf"], points=cfg["points"][q],
+                               family="se", length_scale=cfg["length_scale"][q], diag=cfg["diag"])
+        o.set_LRTolerance(cfg["tol"])
+        xo, _, ro = o.refined_solve(cfg["rhs"][q], target=1e-15, max_steps=4)
+        e = np.linalg.norm(Xs[q] - xo) / np.linalg.norm(xo)
+        lo = o.logAbsDeterminant()
+        print(f"  instance {q} n={cfg['n'][q]}: rel. diff {e:.2e} (oracle residual {ro:.1e}), "
+              f"logdet diff {abs(lds[q] - lo):.1e}", flush=True)
+        # the FP64 preconditioner solves these systems (noise 1e-2) to
+        # rounding; the FP32 one stops at the residual target, and its
+        # forward error is bounded by the condition number times that
+        assert e < (1e-10 if prec == "fp64_refined" else 1e-7)
+        assert abs(lds[q] - lo) <= 1e-12 * max(1.0, abs(lo))
+
+
+def test_predictor_deployed_on_matern_262144():
+    """The predictor on a system outside its training set: Matern-3/2,
+    l = 0.1, N = 262144, tolerance 1e-10.  ACA still decides every rank
+    (identical ranks with and without it); the fraction of blocks whose
+    rank exceeds the predicted capacity stays small, and the predicted
+    sizing does not make compression slower."""
+    _case("matern32 262144 predictor deployment")
+    pts = I.points_2d_kd(262144, 4242)
+    model = RankModel.load()
+    times, nodes, st = {}, {}, None
+    for learned in (False, True, False, True):
+        g = HODLR_Matrix.from_kernel(pts, "matern32", 0.1, 1e-3, sizeThreshold=256)
+        g.set_LRTolerance(1e-10)
+        if learned:
+            g.set_rank_predictor(model, 0.25)
+        g.storeLRinTree()
+        s = g.stats()
+        times.setdefault(learned, []).append(s["compress_time"])
+        nodes[learned] = g.nodes()
+        if learned:
+            st = s
+        del g
+    plain, pred = min(times[False]), min(times[True])
+    miss = st["pred_misses"] / st["pred_blocks"]
+    print(f"compression plain {plain:.4f} s, predictor-sized {pred:.4f} s "
+          f"({plain / pred:.3f}x); misses {st['pred_misses']}/{st['pred_blocks']}, mean |log err| "
+          f"{st['pred_log_err']:.3f}", flush=True)
+    assert np.array_equal(nodes[False], nodes[True])
+    assert miss <= 0.02
+    assert pred <= 1.01 * plain

@@ -8,7 +8,7 @@ import sys
 import numpy as np
 import pytest
 
-from dense_hodlr_amd import HODLR_Matrix, memory_cache, memory_stats, memory_trim
+from dense_hodlr_amd import HODLR_Matrix, HodlrNotConverged, memory_cache, memory_stats, memory_trim
 from dense_hodlr_amd import inputs as I
 
 pytestmark = pytest.mark.gpu
@@ -23,7 +23,10 @@ def _run(pts, b, precision):
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=128)
     g.set_LRTolerance(1e-10)
     g.set_precision(precision)
-    x = g.recSM_Solve(b)
+    try:
+        x = g.recSM_Solve(b)
+    except HodlrNotConverged as e:  # the best iterate: what matters here is reuse
+        x = e.x
     ld = g.logAbsDeterminant()
     nodes = g.nodes()
     del g

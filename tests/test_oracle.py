@@ -334,3 +334,54 @@ def test_aca_subnormal_pivots_count_as_zero(oracle):
     B = np.outer(np.linspace(1e-150, 2e-150, 7), np.linspace(1.0, 3.0, 5))
     W, V, r, _, _ = oracle.aca(B, 1e-10)
     assert r == 1 and np.allclose(W @ V.T, B, rtol=1e-14, atol=0)
+
+
+# ---------------------------------------------------------------- refinement
+def test_oracle_dd_residual_matches_exact(oracle):
+    """The oracle's double-double residual against rational arithmetic on
+    the FP64 operator (its leaves and factors), where the FP64 evaluation
+    loses most digits to cancellation."""
+    from exact_residual import exact_residual
+    n, leaf = 160, 20
+    pts = np.sort(I.points_1d_sorted(n, 13).ravel())
+    d = pts[:, None] - pts[None, :]
+    A = np.exp(-(d / 0.1) ** 2) + 1e-7 * np.eye(n)
+    o = oracle.OracleHODLR(n, leaf=leaf, dense=A)
+    o.set_LRTolerance(1e-12)
+    B = I.rhs_normal(n, 2, 14)
+    X = o.recSM_Solve(B)
+    R = o.residual_dd(B, X)
+    Rx = exact_residual(o, A, B, X)
+    R64 = B - o.matvec(X)
+    scale = np.max(np.abs(Rx))
+    err_dd, err_64 = np.max(np.abs(R - Rx)), np.max(np.abs(R64 - Rx))
+    assert err_dd <= 4 * np.finfo(float).eps * scale
+    assert err_64 > 100 * err_dd
+
+
+def test_oracle_refinement_converges_to_rounding(oracle):
+    """Refinement with double-double residuals drives the recSM solution of
+    an ill-conditioned system (Gaussian, l = 0.1, shift 1e-8: the unrefined
+    solution carries errors near kappa * eps) to the solution of A_H x = b
+    rounded: its forward error against a high-precision solve of the
+    assembled A_H is a few eps, far below the unrefined error."""
+    n, leaf = 256, 32
+    pts = np.sort(I.points_1d_sorted(n, 15).ravel())
+    d = pts[:, None] - pts[None, :]
+    A = np.exp(-(d / 0.1) ** 2) + 1e-8 * np.eye(n)
+    o = oracle.OracleHODLR(n, leaf=leaf, dense=A)
+    o.set_LRTolerance(1e-12)
+    b = I.rhs_normal(n, 1, 16)
+    x0 = o.recSM_Solve(b)
+    x, steps, res = o.refined_solve(b, target=1e-30, max_steps=6)
+    # reference solution of the assembled operator in extended precision:
+    # the residual of x0 in double-double, corrected by a dense LU solve
+    AH = o.dense_operator()
+    xr = x0.copy()
+    for _ in range(4):
+        xr = xr + np.linalg.solve(AH, o.residual_dd(b, xr))
+    e0 = np.linalg.norm(x0 - xr) / np.linalg.norm(xr)
+    e1 = np.linalg.norm(x - xr) / np.linalg.norm(xr)
+    assert steps == 6
+    assert e0 > 1e-9, e0  # the unrefined solution is far off here
+    assert e1 < 1e-14, (e1, e0)
