@@ -34,7 +34,7 @@ EXPORTS = [
     "hodlr_set_lr_method", "hodlr_set_tolerance", "hodlr_set_min_pivot", "hodlr_set_precision",
     "hodlr_set_refinement", "hodlr_set_profiling", "hodlr_set_graphs", "hodlr_autotune", "hodlr_set_launch_shapes", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
     "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
-    "hodlr_matvec", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
+    "hodlr_matvec", "hodlr_matvec_device_noalloc", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
     "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
     "hodlr_create_batch", "hodlr_batch_count", "hodlr_batch_offsets", "hodlr_logabsdet_batch",
     "hodlr_rank_model_create", "hodlr_rank_model_destroy", "hodlr_rank_model_num_params",
@@ -154,6 +154,8 @@ def lib():
         "hodlr_logabsdet": (C.c_int, [h, dp]),
         "hodlr_determinant": (C.c_int, [h, dp]),
         "hodlr_matvec": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_matvec_device_noalloc": (C.c_int, [h, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                                  C.c_int]),
         "hodlr_residual": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int, dp, C.c_int]),
         "hodlr_get_stats": (C.c_int, [h, C.POINTER(Stats)]),
         "hodlr_rows": (C.c_int, [h]),

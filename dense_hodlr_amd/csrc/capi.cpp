@@ -321,6 +321,15 @@ int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, 
   });
 }
 
+int hodlr_matvec_device_noalloc(hodlr_handle h, const double* dX, int ldx, int nrhs, double* dY,
+                                int ldy) {
+  return guard(h, [&] {
+    const int n = h->eng->rows();
+    if (!dX || !dY || nrhs < 0 || ldx < n || ldy < n) throw std::invalid_argument("bad arguments");
+    h->eng->matvec_noalloc(dX, ldx, nrhs, dY, ldy);
+  });
+}
+
 int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const double* X, int ldx,
                    double* R, int ldr) {
   return guard(h, [&] {

@@ -90,6 +90,7 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   overlap_root_ = std::getenv("HODLR_NO_ROOT_OVERLAP") == nullptr;
   build_trees();
   upload_sources();
+  noalloc_setup();  // the allocation-free product's scratch address on this device
   // problem shape: instance sizes, leaf sizes and kinds
   shape_key_ = std::to_string(specs_.size());
   for (auto& sp : specs_)

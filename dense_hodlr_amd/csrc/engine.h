@@ -36,6 +36,9 @@ namespace hodlr {
 
 constexpr int LU_NB = 64;  // largest panel width of the blocked LU and triangular solves
 
+/* Size of the per-GPU device array behind Engine::matvec_noalloc. */
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
+
 /* Launch shapes chosen by the autotuner (defaults without it). */
 struct Tuning {
   int aca_pt = 2;          // panel rows per thread of the ACA row/column kernels (1, 2, 4)
@@ -194,6 +197,11 @@ class Engine {
   void solve_host(int which, const double* B, int ldb, int nrhs, double* X, int ldx);
   double logabsdet(double* det_prod);
   void matvec_device(const double* dX, int ldx, int nrhs, double* dY, int ldy);
+  /* The same product with no device allocation: temporaries in the per-GPU
+   * scratch array (NOALLOC_SCRATCH_BYTES); needs a compressed operator. */
+  void matvec_noalloc(const double* dX, int ldx, int nrhs, double* dY, int ldy);
+  /* Looks up the current device's scratch address (once per device). */
+  static void noalloc_setup();
   void residual_dd(const double* dB, int ldb, const double* dX, int ldx, int nrhs, double* dR,
                    int ldr);
 

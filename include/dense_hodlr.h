@@ -229,6 +229,19 @@ int hodlr_determinant(hodlr_handle h, double* out);
  * operator that fill_Block reconstructs, HODLR_Matrix.cpp:1404-1480). */
 int hodlr_matvec(hodlr_handle h, const double* X, int ldx, int nrhs, double* Y, int ldy);
 
+/* Y = A_H X on device pointers with no device allocation: no counterpart in
+ * the reference; for callers that cannot allocate (code under stream
+ * capture, serving loops, harnesses that forbid hipMalloc).  The result is
+ * bit-identical to hodlr_matvec's.  Every temporary (projections V^T X of a
+ * level, split-K partials, launch descriptors) lives in one 64 MiB device
+ * array per GPU, held until the process ends: calls whose temporaries need
+ * more (many right-hand sides on a large operator) return HODLR_ERR_ARG.
+ * The operator must be compressed already (hodlr_compress), else
+ * HODLR_ERR_STATE.  Calls that overlap on one GPU share that array and race:
+ * serialize them. */
+int hodlr_matvec_device_noalloc(hodlr_handle h, const double* dX, int ldx, int nrhs, double* dY,
+                                int ldy);
+
 /* R = B - A_H X evaluated in double-double arithmetic (error-free products
  * and sums) and rounded once: the residual that refinement targets are
  * checked with.  No reference counterpart. */
