@@ -35,13 +35,16 @@ EXPORTS = [
     "hodlr_set_refinement", "hodlr_set_profiling", "hodlr_set_graphs", "hodlr_autotune", "hodlr_set_launch_shapes", "hodlr_compress", "hodlr_factorize", "hodlr_solve_sm",
     "hodlr_solve_lu", "hodlr_solve_device", "hodlr_logabsdet", "hodlr_determinant",
     "hodlr_matvec", "hodlr_matvec_device_noalloc", "hodlr_residual", "hodlr_get_stats", "hodlr_rows", "hodlr_num_nodes", "hodlr_get_nodes",
-    "hodlr_get_factor", "hodlr_last_error", "hodlr_version",
+    "hodlr_get_factor", "hodlr_block", "hodlr_block_device", "hodlr_last_error", "hodlr_version",
     "hodlr_create_batch", "hodlr_batch_count", "hodlr_batch_offsets", "hodlr_logabsdet_batch",
     "hodlr_rank_model_create", "hodlr_rank_model_destroy", "hodlr_rank_model_num_params",
     "hodlr_rank_model_get_params", "hodlr_rank_model_set_params", "hodlr_rank_model_forward",
     "hodlr_rank_model_predict_bf16", "hodlr_rank_model_gradient", "hodlr_rank_model_train",
     "hodlr_block_features", "hodlr_set_rank_predictor",
     "hodlr_memory_get_stats", "hodlr_memory_trim", "hodlr_memory_cache",
+    "hodlr_read_binary_matrix", "hodlr_write_binary_matrix", "hodlr_read_text_matrix",
+    "hodlr_create_dense_from_file", "hodlr_create_kernel_tree", "hodlr_create_dense_tree",
+    "hodlr_kd_sort", "hodlr_kd_tree", "hodlr_diag_block", "hodlr_split_at_top",
 ]
 
 
@@ -162,6 +165,9 @@ def lib():
         "hodlr_num_nodes": (C.c_int, [h]),
         "hodlr_get_nodes": (C.c_int, [h, ip, ip, ip, ip, ip, ip]),
         "hodlr_get_factor": (C.c_int, [h, C.c_int, C.c_int, dp]),
+        "hodlr_block": (C.c_int, [h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_block_device": (C.c_int, [h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                         C.c_void_p, C.c_int]),
         "hodlr_last_error": (C.c_char_p, [h]),
         "hodlr_version": (C.c_char_p, []),
         "hodlr_create_batch": (C.c_int, [hp, C.c_int, ip, C.POINTER(KernelDesc), C.c_int]),
@@ -180,6 +186,17 @@ def lib():
                                              C.c_ulonglong, fp]),
         "hodlr_block_features": (C.c_int, [h, C.c_double, fp]),
         "hodlr_set_rank_predictor": (C.c_int, [h, h, C.c_double]),
+        "hodlr_read_binary_matrix": (C.c_int, [C.c_char_p, ip, ip, dp, C.c_int]),
+        "hodlr_write_binary_matrix": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, C.c_int]),
+        "hodlr_read_text_matrix": (C.c_int, [C.c_char_p, ip, ip, dp, C.c_int]),
+        "hodlr_create_dense_from_file": (C.c_int, [hp, C.c_char_p, C.c_int, C.c_int]),
+        "hodlr_create_kernel_tree": (C.c_int, [hp, C.c_int, C.POINTER(KernelDesc), C.c_int, ip,
+                                               C.c_int]),
+        "hodlr_create_dense_tree": (C.c_int, [hp, C.c_int, dp, C.c_int, C.c_int, ip, C.c_int]),
+        "hodlr_kd_sort": (C.c_int, [dp, C.c_int, C.c_int, ip]),
+        "hodlr_kd_tree": (C.c_int, [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, ip]),
+        "hodlr_diag_block": (C.c_int, [h, C.c_int, hp]),
+        "hodlr_split_at_top": (C.c_int, [h, hp, hp]),
         "hodlr_memory_get_stats": (C.c_int, [C.POINTER(MemoryStats)]),
         "hodlr_memory_trim": (C.c_int, []),
         "hodlr_memory_cache": (C.c_int, [C.c_int]),

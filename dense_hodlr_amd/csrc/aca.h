@@ -35,7 +35,7 @@ struct AcaLaunch {
   int n_row_wg, n_col_wg, n_fin_wg;
   int jpt, ipt;
   double tol;
-  int min_rank, max_rank, min_pivot;
+  int min_pivot;
 };
 
 void aca_launch_init(const AcaBlock* d_blocks, AcaState* d_state, uint32_t* d_bits, int nblocks,

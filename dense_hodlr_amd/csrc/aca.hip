@@ -256,7 +256,7 @@ __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, A
   AcaState s;
   s.k = 0;
   s.cur_row = 0;
-  s.status = (blocks[b].m > 0 && blocks[b].n > 0) ? 0 : 1;
+  s.status = (blocks[b].m > 0 && blocks[b].n > 0 && !blocks[b].skip) ? 0 : 1;
   s.cur_col = -1;
   s.frob2 = 0.0;
   s.eps = 1.0;
@@ -416,8 +416,7 @@ __global__ void __launch_bounds__(TPB)
     aca_fin_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    const int* __restrict__ wg2chunk, const int* __restrict__ nchunk,
                    AcaState* __restrict__ st, uint32_t* __restrict__ bits,
-                   double* __restrict__ fin_part, int* __restrict__ counters, double tol,
-                   int min_rank, int max_rank) {
+                   double* __restrict__ fin_part, int* __restrict__ counters, double tol) {
   __shared__ double red[TPB / 64];
   __shared__ double part[4][FIN_L];
   __shared__ PivRed pr_sh[TPB / 64];
@@ -518,7 +517,7 @@ __global__ void __launch_bounds__(TPB)
     // complete epsilon of term k-1 (:238-241) and the loop test (:149)
     ns.frob2 = s.frob2 + 2.0 * crossSum + nw2 * nv2;
     ns.eps = sqrt(nw2) * sqrt(nv2) / sqrt(ns.frob2);
-    if (!((ns.eps > tol) || (k < min_rank))) {
+    if (!((ns.eps > tol) || (k < b.min_rank))) {
       ns.status = 1;  // term k-1 was the last; this iteration is discarded
       st[bidx] = ns;
       return;
@@ -554,7 +553,7 @@ __global__ void __launch_bounds__(TPB)
   ns.cur_row = next;
   bit_set(rowbits, next);
   const int upper = min(b.m, b.n);
-  if (ns.k == max_rank || ns.k >= upper) {
+  if (ns.k == b.max_rank || ns.k >= upper) {
     ns.status = 1;  // the loop ends whatever epsilon is (:246-247, :149)
   } else {
     ns.pending = 1;
@@ -573,7 +572,11 @@ __global__ void __launch_bounds__(TPB)
                     const SRC* __restrict__ srcs, int* __restrict__ ranks) {
   const int bidx = blockIdx.x;
   const AcaBlock b = blocks[bidx];
-  const int k = st[bidx].k;
+  if (b.skip == 2) {  // compressed by another method
+    if (threadIdx.x == 0) ranks[bidx] = -1;
+    return;
+  }
+  const int k = st[bidx].k;  // (0 for max_rank 0: one zero column below)
   const int ld = L.ld;
   const int cap = L.cap[b.level];
   double* Wp = L.W[b.level] + b.w_row0;
@@ -657,8 +660,7 @@ void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySou
   HODLR_HIP_CHECK(hipGetLastError());
   if (ev) HODLR_HIP_CHECK(hipEventRecord(ev[2], s));
   aca_fin_kernel<<<a.n_fin_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_fin, a.wg2chunk_fin, a.nchunk,
-                                             a.state, a.bits, a.fin_part, a.counters, a.tol,
-                                             a.min_rank, a.max_rank);
+                                             a.state, a.bits, a.fin_part, a.counters, a.tol);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

@@ -11,6 +11,8 @@
 
 #include "../../include/dense_hodlr.h"
 #include "engine.h"
+#include "kd.h"
+#include "matrix_io.h"
 #include "model.h"
 
 struct hodlr_handle_s {
@@ -74,6 +76,21 @@ int guard_model(hodlr_rank_model m, F&& f) {
   }
 }
 
+/* Host-only calls (matrix files): errors go to hodlr_last_error(NULL). */
+template <class F>
+int guard_io(F&& f) {
+  try {
+    f();
+    return HODLR_OK;
+  } catch (const std::invalid_argument& e) {
+    return fail(nullptr, HODLR_ERR_ARG, e.what());
+  } catch (const std::bad_alloc&) {
+    return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
+  } catch (const std::exception& e) {
+    return fail(nullptr, HODLR_ERR_STATE, e.what());
+  }
+}
+
 int check_device() {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n < 1) return -1;
@@ -86,6 +103,9 @@ int make_handle(hodlr_handle* h, std::vector<hodlr::InstanceSpec>&& specs) {
   if (!hh) return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
   try {
     hh->eng.reset(new hodlr::Engine(std::move(specs)));
+  } catch (const std::invalid_argument& e) {  // (a user partition that does not fit)
+    delete hh;
+    return fail(nullptr, HODLR_ERR_ARG, e.what());
   } catch (const std::exception& e) {
     delete hh;
     return fail(nullptr, HODLR_ERR_DEVICE, e.what());
@@ -119,6 +139,33 @@ std::string kernel_spec(int n, const hodlr_kernel_desc* k, int leaf, hodlr::Inst
   return std::string();
 }
 
+/* A user partition from the C node array. */
+std::vector<hodlr::UserNode> user_tree(const hodlr_tree_node* nodes, int count) {
+  std::vector<hodlr::UserNode> t;
+  if (count < 0 || (count > 0 && !nodes)) throw std::invalid_argument("bad tree arguments");
+  for (int i = 0; i < count; i++) {
+    hodlr::UserNode u;
+    u.split = nodes[i].split;
+    u.min_rank[0] = nodes[i].top_min_rank;
+    u.min_rank[1] = nodes[i].bottom_min_rank;
+    u.max_rank[0] = nodes[i].top_max_rank;
+    u.max_rank[1] = nodes[i].bottom_max_rank;
+    u.lr = nodes[i].lr_method;
+    u.left = nodes[i].left;
+    u.right = nodes[i].right;
+    t.push_back(u);
+  }
+  return t;
+}
+
+int wrap_engine(hodlr_handle* out, std::unique_ptr<hodlr::Engine> e) {
+  auto* hh = new (std::nothrow) hodlr_handle_s();
+  if (!hh) return fail(nullptr, HODLR_ERR_NOMEM, "host allocation failed");
+  hh->eng = std::move(e);
+  *out = hh;
+  return HODLR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -130,6 +177,92 @@ int hodlr_create_kernel(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int 
   const std::string err = kernel_spec(n, k, leaf, specs[0]);
   if (!err.empty()) return fail(nullptr, HODLR_ERR_ARG, err);
   return make_handle(h, std::move(specs));
+}
+
+int hodlr_create_kernel_tree(hodlr_handle* h, int n, const hodlr_kernel_desc* k, int leaf,
+                             const hodlr_tree_node* nodes, int count) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  std::vector<hodlr::InstanceSpec> specs(1);
+  const std::string err = kernel_spec(n, k, leaf, specs[0]);
+  if (!err.empty()) return fail(nullptr, HODLR_ERR_ARG, err);
+  int rc = guard_io([&] { specs[0].tree = user_tree(nodes, count); });
+  if (rc != HODLR_OK) return rc;
+  return make_handle(h, std::move(specs));
+}
+
+int hodlr_create_dense_tree(hodlr_handle* h, int n, const double* A, int lda, int leaf,
+                            const hodlr_tree_node* nodes, int count) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (n <= 0 || leaf <= 0 || !A || lda < n) return fail(nullptr, HODLR_ERR_ARG, "bad arguments");
+  std::vector<hodlr::InstanceSpec> specs(1);
+  hodlr::InstanceSpec& s = specs[0];
+  s.n = n;
+  s.leaf = leaf;
+  s.kind = 1;
+  int rc = guard_io([&] {
+    s.tree = user_tree(nodes, count);
+    s.dense.resize((size_t)n * n);
+    for (int j = 0; j < n; j++)
+      std::memcpy(s.dense.data() + (size_t)j * n, A + (size_t)j * lda, (size_t)n * sizeof(double));
+  });
+  if (rc != HODLR_OK) return rc;
+  return make_handle(h, std::move(specs));
+}
+
+int hodlr_kd_sort(const double* points, int npoints, int dim, int* perm) {
+  return guard_io([&] {
+    if (!perm) throw std::invalid_argument("null output");
+    hodlr::kd_sort(points, npoints, dim, perm);
+  });
+}
+
+int hodlr_kd_tree(const double* points, int npoints, int dim, int unknowns_per_point, int max_rank,
+                  int lr_method, int* perm, hodlr_tree_node* nodes, int* count) {
+  return guard_io([&] {
+    if (!perm || !nodes || !count) throw std::invalid_argument("null output");
+    const std::vector<hodlr::UserNode> t =
+        hodlr::kd_tree(points, npoints, dim, unknowns_per_point, max_rank, lr_method, perm);
+    for (size_t i = 0; i < t.size(); i++) {
+      nodes[i].split = t[i].split;
+      nodes[i].top_min_rank = t[i].min_rank[0];
+      nodes[i].bottom_min_rank = t[i].min_rank[1];
+      nodes[i].top_max_rank = t[i].max_rank[0];
+      nodes[i].bottom_max_rank = t[i].max_rank[1];
+      nodes[i].lr_method = t[i].lr;
+      nodes[i].left = t[i].left;
+      nodes[i].right = t[i].right;
+    }
+    *count = (int)t.size();
+  });
+}
+
+int hodlr_diag_block(hodlr_handle h, int side, hodlr_handle* out) {
+  if (!out) return fail(h, HODLR_ERR_ARG, "null output handle");
+  *out = nullptr;
+  std::unique_ptr<hodlr::Engine> e;
+  const int rc = guard(h, [&] { e = h->eng->diagonal_block(side); });
+  if (rc != HODLR_OK) return rc;
+  return wrap_engine(out, std::move(e));
+}
+
+int hodlr_split_at_top(hodlr_handle h, hodlr_handle* top, hodlr_handle* bottom) {
+  if (!top || !bottom) return fail(h, HODLR_ERR_ARG, "null output handle");
+  *top = *bottom = nullptr;
+  std::unique_ptr<hodlr::Engine> t, b;
+  const int rc = guard(h, [&] {
+    t = h->eng->diagonal_block(0);
+    b = h->eng->diagonal_block(1);
+  });
+  if (rc != HODLR_OK) return rc;
+  int r = wrap_engine(top, std::move(t));
+  if (r == HODLR_OK) r = wrap_engine(bottom, std::move(b));
+  if (r != HODLR_OK) {
+    hodlr_destroy(*top);
+    *top = nullptr;
+  }
+  return r;
 }
 
 int hodlr_create_batch(hodlr_handle* h, int count, const int* n, const hodlr_kernel_desc* kernels,
@@ -209,6 +342,61 @@ int hodlr_create_dense(hodlr_handle* h, int n, const double* A, int lda, int lea
   for (int j = 0; j < n; j++)
     std::memcpy(s.dense.data() + (size_t)j * n, A + (size_t)j * lda, (size_t)n * sizeof(double));
   int rc = make_handle(h, std::move(specs));
+  if (rc != HODLR_OK) return rc;
+  rc = hodlr_set_lr_method(*h, lr_method);
+  if (rc != HODLR_OK) {
+    g_create_err = (*h)->err;
+    hodlr_destroy(*h);
+    *h = nullptr;
+  }
+  return rc;
+}
+
+/* ---- matrix files ---- */
+int hodlr_read_binary_matrix(const char* path, int* rows, int* cols, double* out, int ld) {
+  return guard_io([&] {
+    if (!path || !rows || !cols) throw std::invalid_argument("null argument");
+    hodlr::binary_matrix_dims(path, rows, cols);
+    if (out) hodlr::read_binary_matrix(path, *rows, *cols, out, ld);
+  });
+}
+
+int hodlr_write_binary_matrix(const char* path, int rows, int cols, const double* A, int lda) {
+  return guard_io([&] {
+    if (!path || (!A && rows > 0 && cols > 0)) throw std::invalid_argument("null argument");
+    hodlr::write_binary_matrix(path, rows, cols, A, lda);
+  });
+}
+
+int hodlr_read_text_matrix(const char* path, int* rows, int* cols, double* out, int ld) {
+  return guard_io([&] {
+    if (!path || !rows || !cols) throw std::invalid_argument("null argument");
+    hodlr::text_matrix_dims(path, rows, cols);
+    if (out) hodlr::read_text_matrix(path, *rows, *cols, out, ld);
+  });
+}
+
+int hodlr_create_dense_from_file(hodlr_handle* h, const char* path, int leaf, int lr_method) {
+  if (!h) return fail(nullptr, HODLR_ERR_ARG, "null handle pointer");
+  *h = nullptr;
+  if (!path || leaf <= 0) return fail(nullptr, HODLR_ERR_ARG, "bad arguments");
+  if (lr_method != HODLR_LR_ACA && lr_method != HODLR_LR_SVD)
+    return fail(nullptr, HODLR_ERR_ARG, "unknown low-rank method");
+  std::vector<hodlr::InstanceSpec> specs(1);
+  hodlr::InstanceSpec& s = specs[0];
+  int rows = 0, cols = 0;
+  int rc = guard_io([&] {
+    hodlr::binary_matrix_dims(path, &rows, &cols);
+    // HODLR_Matrix.cpp:92: assert(isSquareMatrix == true)
+    if (rows != cols || rows <= 0) throw std::invalid_argument("the HODLR matrix must be square");
+    s.dense.resize((size_t)rows * rows);
+    hodlr::read_binary_matrix(path, rows, cols, s.dense.data(), rows);
+  });
+  if (rc != HODLR_OK) return rc;
+  s.n = rows;
+  s.leaf = leaf;
+  s.kind = 1;
+  rc = make_handle(h, std::move(specs));
   if (rc != HODLR_OK) return rc;
   rc = hodlr_set_lr_method(*h, lr_method);
   if (rc != HODLR_OK) {
@@ -480,6 +668,21 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->aca_col_time = s.aca_col_time;
     out->aca_launches = s.aca_launches;
     out->aca_bytes = s.aca_bytes;
+  });
+}
+
+int hodlr_block(hodlr_handle h, int which, int i0, int j0, int m, int n, double* out, int ldo) {
+  return guard(h, [&] {
+    if (!out && m > 0 && n > 0) throw std::invalid_argument("null output");
+    h->eng->block_host(which, i0, j0, m, n, out, ldo);
+  });
+}
+
+int hodlr_block_device(hodlr_handle h, int which, int i0, int j0, int m, int n, double* dout,
+                       int ldo) {
+  return guard(h, [&] {
+    if (!dout && m > 0 && n > 0) throw std::invalid_argument("null output");
+    h->eng->block_device(which, i0, j0, m, n, dout, ldo);
   });
 }
 

@@ -64,6 +64,10 @@ struct AcaBlock {
   int bits_col0;        /* word offset of the chosen-column bitmap */
   int part_off;         /* offset (in doubles) of this block's partial slots */
   int lidx;             /* index of the block within its level (Gram slot) */
+  int min_rank;         /* the node's rank bounds (-1: none; max_rank 0: a zero */
+  int max_rank;         /* rank-1 block without iterating, lowRank.cpp:120-125) */
+  int skip;             /* 0 ACA iterates; 1 max_rank 0 (no iterations); 2 another method */
+  int pad_;
 };
 
 /* Per-block iteration state of ACA (device resident). */

@@ -61,6 +61,16 @@ struct AssembleDesc {
   int pad_;
 };
 
+/* out(i, j) = sum_l U(i, l) V(j, l) over an m x n piece of a low-rank block
+ * (U m x k with ld ldu, V n x k with ld ldv; out ld ldo). */
+struct LrFillDesc {
+  const double* U;
+  const double* V;
+  double* out;
+  int m, n, k;
+  int ldu, ldv, ldo;
+};
+
 /* Copy of an m x n column-major block. */
 struct CopyDesc {
   const double* src;
@@ -164,6 +174,8 @@ void launch_assemble(const AssembleDesc* d, int count, int max_elems, const Entr
                      hipStream_t s);
 void launch_set_identity(const CopyDesc* d, int count, hipStream_t s);
 void launch_copy(const CopyDesc* d, int count, hipStream_t s);
+/* every piece of the launch (max_elems: the largest m * n) */
+void launch_lr_fill(const LrFillDesc* d, int count, size_t max_elems, hipStream_t s);
 void launch_logdet(const LuDesc* d, int count, double* out_log, double* out_prod, hipStream_t s);
 template <class TZ>
 void launch_capacitance(const CapDescT<TZ>* d, int count, hipStream_t s);

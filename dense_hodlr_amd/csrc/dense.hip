@@ -27,6 +27,18 @@ __global__ void __launch_bounds__(TPB)
   }
 }
 
+/* entries of low-rank pieces, the product summed sequentially over l */
+__global__ void __launch_bounds__(TPB) lr_fill_kernel(const LrFillDesc* __restrict__ d) {
+  const LrFillDesc a = d[blockIdx.y];
+  const size_t total = (size_t)a.m * a.n;
+  for (size_t e = (size_t)blockIdx.x * TPB + threadIdx.x; e < total; e += (size_t)gridDim.x * TPB) {
+    const int i = (int)(e % a.m), j = (int)(e / a.m);
+    double acc = 0.0;
+    for (int l = 0; l < a.k; l++) acc += a.U[(size_t)i + (size_t)l * a.ldu] * a.V[(size_t)j + (size_t)l * a.ldv];
+    a.out[(size_t)i + (size_t)j * a.ldo] = acc;
+  }
+}
+
 __global__ void __launch_bounds__(TPB) identity_kernel(const CopyDesc* __restrict__ d) {
   const CopyDesc c = d[blockIdx.y];
   const size_t tot = (size_t)c.m * c.n;
@@ -1299,6 +1311,16 @@ void launch_set_identity(const CopyDesc* d, int count, hipStream_t s) {
   for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
     dim3 g(16, std::min(MAX_GRID_Y, count - c0));
     identity_kernel<<<g, TPB, 0, s>>>(d + c0);
+    HODLR_HIP_CHECK(hipGetLastError());
+  }
+}
+
+void launch_lr_fill(const LrFillDesc* d, int count, size_t max_elems, hipStream_t s) {
+  if (count == 0) return;
+  for (int c0 = 0; c0 < count; c0 += MAX_GRID_Y) {
+    const int c = std::min(MAX_GRID_Y, count - c0);
+    dim3 g(grid_x(max_elems, count), c);
+    lr_fill_kernel<<<g, TPB, 0, s>>>(d + c0);
     HODLR_HIP_CHECK(hipGetLastError());
   }
 }

@@ -44,6 +44,11 @@ T* DescArena::push(const std::vector<T>& v, hipStream_t s) {
   return reinterpret_cast<T*>(dst);
 }
 
+// (push is used by other files of the engine for these descriptor types)
+template AssembleDesc* DescArena::push<AssembleDesc>(const std::vector<AssembleDesc>&, hipStream_t);
+template CopyDesc* DescArena::push<CopyDesc>(const std::vector<CopyDesc>&, hipStream_t);
+template LrFillDesc* DescArena::push<LrFillDesc>(const std::vector<LrFillDesc>&, hipStream_t);
+
 void DescArena::reset() {
   for (auto& c : chunks_) c.used = 0;
   while (chunks_.size() > 1) chunks_.pop_front();
@@ -204,9 +209,15 @@ void Engine::finish_root() {
   aux_arena_.reset();
 }
 
-/* Default tree of each instance (src/HODLR_Tree.cpp:33-132): the root splits
- * at n/2-1; a left child of size s splits at min + s/2 - 1, a right child at
- * max - s/2; the root is a leaf if n < leaf, a child if s <= leaf. */
+/* Tree of each instance.  Default (src/HODLR_Tree.cpp:33-132): the root
+ * splits at n/2-1; a left child of size s splits at min + s/2 - 1, a right
+ * child at max - s/2; the root is a leaf if n < leaf, a child if s <= leaf.
+ * User partition (userTree_To_HODLRTree, HODLR_Tree.cpp:207-321): a user node
+ * with split -1 and at most leaf rows is a leaf, any other takes its split,
+ * rank bounds and method; a user node without children gets default nodes
+ * below it, the left one split at min + s/2 - 1 and the right one at
+ * split + s/2 (not max - s/2: the reference's rule there), each with a
+ * default subtree. */
 void Engine::build_trees() {
   struct Rec {
     int level, lo, sp, hi;
@@ -222,33 +233,96 @@ void Engine::build_trees() {
     const int row0 = ntot_;
     inst_row0_.push_back(row0);
     ntot_ += n;
-    if (n < leaf) {
-      lv.push_back({q, 0, 0, n - 1, row0, n, 0, 0});
-      nlev_ = std::max(nlev_, 1);
-      continue;
-    }
-    std::vector<Rec> stack{{0, 0, n / 2 - 1, n - 1}};
-    while (!stack.empty()) {
-      Rec r = stack.back();
-      stack.pop_back();
+    auto add_leaf = [&](int level, int lo, int hi) {
+      lv.push_back({q, level, lo, hi, row0 + lo, hi - lo + 1, 0, 0});
+      nlev_ = std::max(nlev_, level + 1);
+    };
+    auto add_node = [&](int level, int lo, int split, int hi) -> TreeNodeH& {
       TreeNodeH nd;
       nd.inst = q;
-      nd.level = r.level;
-      nd.min_i = r.lo;
-      nd.split = r.sp;
-      nd.max_i = r.hi;
-      nd.row0 = row0 + r.lo;
+      nd.level = level;
+      nd.min_i = lo;
+      nd.split = split;
+      nd.max_i = hi;
+      nd.row0 = row0 + lo;
       all.push_back(nd);
-      const int ts = r.sp - r.lo + 1, bs = r.hi - r.sp;
-      nlev_ = std::max(nlev_, r.level + 2);
-      if (ts > leaf)
-        stack.push_back({r.level + 1, r.lo, r.lo + ts / 2 - 1, r.sp});
-      else
-        lv.push_back({q, r.level + 1, r.lo, r.sp, row0 + r.lo, ts, 0, 0});
-      if (bs > leaf)
-        stack.push_back({r.level + 1, r.sp + 1, r.hi - bs / 2, r.hi});
-      else
-        lv.push_back({q, r.level + 1, r.sp + 1, r.hi, row0 + r.sp + 1, bs, 0, 0});
+      nlev_ = std::max(nlev_, level + 2);
+      return all.back();
+    };
+    std::vector<Rec> stack;  // non-leaf nodes whose children follow the default rules
+    auto default_below = [&]() {
+      while (!stack.empty()) {
+        Rec r = stack.back();
+        stack.pop_back();
+        add_node(r.level, r.lo, r.sp, r.hi);
+        const int ts = r.sp - r.lo + 1, bs = r.hi - r.sp;
+        if (ts > leaf)
+          stack.push_back({r.level + 1, r.lo, r.lo + ts / 2 - 1, r.sp});
+        else
+          add_leaf(r.level + 1, r.lo, r.sp);
+        if (bs > leaf)
+          stack.push_back({r.level + 1, r.sp + 1, r.hi - bs / 2, r.hi});
+        else
+          add_leaf(r.level + 1, r.sp + 1, r.hi);
+      }
+    };
+    if (sp.tree.empty()) {
+      if (n < leaf) {
+        add_leaf(0, 0, n - 1);
+        continue;
+      }
+      stack.push_back({0, 0, n / 2 - 1, n - 1});
+      default_below();
+      continue;
+    }
+    // user partition: depth-first over the user nodes
+    struct URec {
+      int u, level, lo, hi;
+    };
+    std::vector<URec> us{{0, 0, 0, n - 1}};
+    size_t visited = 0;
+    while (!us.empty()) {
+      const URec r = us.back();
+      us.pop_back();
+      if (r.u < 0 || r.u >= (int)sp.tree.size() || ++visited > sp.tree.size())
+        throw std::invalid_argument("user tree: bad child index or a cycle");
+      const UserNode& u = sp.tree[r.u];
+      const int size = r.hi - r.lo + 1;
+      if (size <= leaf && u.split == -1) {
+        add_leaf(r.level, r.lo, r.hi);
+        continue;
+      }
+      if (u.split == -1)
+        throw std::invalid_argument("user tree: a node of " + std::to_string(size) +
+                                    " rows (more than the leaf size) has no split");
+      if (u.split < r.lo || u.split >= r.hi)
+        throw std::invalid_argument("user tree: split " + std::to_string(u.split) +
+                                    " outside its node [" + std::to_string(r.lo) + ", " +
+                                    std::to_string(r.hi) + "]");
+      if (u.lr < -1 || u.lr > 1) throw std::invalid_argument("user tree: unknown low-rank method");
+      TreeNodeH& nd = add_node(r.level, r.lo, u.split, r.hi);
+      nd.lr = u.lr;
+      for (int h = 0; h < 2; h++) {
+        nd.min_rank[h] = u.min_rank[h];
+        nd.max_rank[h] = u.max_rank[h];
+      }
+      if (u.left < 0 && u.right < 0) {
+        const int ls = u.split - r.lo + 1, rs = r.hi - u.split;
+        if (ls <= leaf)
+          add_leaf(r.level + 1, r.lo, u.split);
+        else
+          stack.push_back({r.level + 1, r.lo, r.lo + ls / 2 - 1, u.split});
+        if (rs <= leaf)
+          add_leaf(r.level + 1, u.split + 1, r.hi);
+        else
+          stack.push_back({r.level + 1, u.split + 1, u.split + rs / 2, r.hi});
+        default_below();
+      } else if (u.left >= 0 && u.right >= 0) {
+        us.push_back({u.right, r.level + 1, u.split + 1, r.hi});
+        us.push_back({u.left, r.level + 1, r.lo, u.split});
+      } else {
+        throw std::invalid_argument("user tree: a node has one child");
+      }
     }
   }
   std::sort(all.begin(), all.end(), [](const TreeNodeH& a, const TreeNodeH& b) {
@@ -339,10 +413,15 @@ void Engine::reset_attributes() {
   factor_time_pending_ = false;
 }
 
+/* set_LRMethod (HODLR_Matrix.cpp:1277-1286, HODLR_Tree.cpp:377-390): a
+ * method other than the handle's default becomes every node's method (the
+ * per-node choices of a user partition included); the default itself is a
+ * no-op, as in the reference. */
 void Engine::set_lr_method(int m) {
   if (m != 0 && m != 1) throw std::invalid_argument("unknown low-rank method");
   if (m != lr_method_) {
     lr_method_ = m;
+    for (auto& nd : nodes_) nd.lr = -1;
     reset_attributes();
   }
 }
@@ -628,10 +707,33 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
 void Engine::compress() {
   if (compressed_) return;
   HODLR_HIP_CHECK(hipEventRecord(ev0_, stream_));
-  if (lr_method_ == 1)
-    svd_compress();
-  else
-    aca_compress();
+  const int nn = (int)nodes_.size();
+  U_.clear();
+  V_.clear();
+  U_.resize(nlev_);
+  V_.resize(nlev_);
+  K_.assign(nlev_, 0);
+  off_.assign(nlev_ + 1, 0);
+  bool any_aca = false, any_svd = false;
+  for (auto& nd : nodes_) {
+    if (node_method(nd) == 1)
+      any_svd = true;
+    else
+      any_aca = true;
+  }
+  std::vector<int> blk_rank(2 * nn, -1);
+  if (any_aca) aca_compress(blk_rank);
+  if (any_svd) svd_compress(blk_rank);
+  for (int id = 0; id < nn; id++) {
+    nodes_[id].kt = blk_rank[2 * id];
+    nodes_[id].kb = blk_rank[2 * id + 1];
+    if (nodes_[id].kt < 1 || nodes_[id].kb < 1) throw std::logic_error("a block was not compressed");
+    const int l = nodes_[id].level;
+    K_[l] = std::max(K_[l], std::max(nodes_[id].kt, nodes_[id].kb));
+  }
+  off_[0] = 0;
+  for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
+  ktot_ = off_[nlev_];
   assemble_leaves();
   HODLR_HIP_CHECK(hipEventRecord(ev1_, stream_));
   HODLR_HIP_CHECK(hipEventSynchronize(ev1_));
@@ -642,19 +744,10 @@ void Engine::compress() {
   compressed_ = true;
 }
 
-void Engine::aca_compress() {
+void Engine::aca_compress(std::vector<int>& blk_rank) {
   const int nn = (int)nodes_.size();
   const int nl = std::max(1, nlev_ - 1);
-  U_.clear();
-  V_.clear();
-  U_.resize(nlev_);
-  V_.resize(nlev_);
-  K_.assign(nlev_, 0);
-  off_.assign(nlev_ + 1, 0);
-  if (nn == 0) {
-    ktot_ = 0;
-    return;
-  }
+  if (nn == 0) return;
   const int jpt = tune_.aca_pt, ipt = tune_.aca_pt;
   const int cpw = 256 * jpt, rpw = 256 * ipt;
   const int nb = 2 * nn;
@@ -690,6 +783,10 @@ void Engine::aca_compress() {
       words += (b.m + 31) / 32;
       b.bits_col0 = (int)words;
       words += (b.n + 31) / 32;
+      b.min_rank = nd.min_rank[h];  // -1 in default trees (HODLR_Tree.cpp:44-47)
+      b.max_rank = nd.max_rank[h];
+      b.skip = node_method(nd) != 0 ? 2 : (nd.max_rank[h] == 0 ? 1 : 0);
+      b.pad_ = 0;
       b.lidx = lvl_count[b.level]++;
       b.part_off = lvl_slots[b.level];
       lvl_slots[b.level] += b.row_nwg + b.col_nwg;
@@ -755,8 +852,9 @@ void Engine::aca_compress() {
   HODLR_HIP_CHECK(hipMemcpyAsync(d_blk.p, blk.data(), nb * sizeof(AcaBlock), hipMemcpyHostToDevice, stream_));
   aca_launch_init(d_blk.p, d_state.p, d_bits.p, nb, stream_);
 
-  std::vector<int> active(nb);
-  for (int b = 0; b < nb; b++) active[b] = b;
+  std::vector<int> active;
+  for (int b = 0; b < nb; b++)
+    if (blk[b].skip == 0 && blk[b].m > 0 && blk[b].n > 0) active.push_back(b);
   std::vector<int> m_row, m_col, m_fin, m_chunk, nchunk(nb, 1);
   AcaLaunch a{};
   auto build_maps = [&]() {
@@ -797,8 +895,6 @@ void Engine::aca_compress() {
     a.jpt = jpt;
     a.ipt = ipt;
     a.tol = tol_;
-    a.min_rank = -1;  // default tree: topOffDiag_minRank = -1 (HODLR_Tree.cpp:44-47)
-    a.max_rank = -1;
     a.min_pivot = min_pivot_;
   };
   build_maps();
@@ -810,7 +906,7 @@ void Engine::aca_compress() {
   long long max_dim = 0;
   for (auto& b : blk) max_dim = std::max<long long>(max_dim, (long long)b.m + b.n);
   long long launched = 0;
-  while (true) {
+  while (!active.empty()) {
     if (launched > max_dim + 64)
       throw std::runtime_error("ACA did not terminate within m + n iterations");
     if (profiling_) {
@@ -880,24 +976,19 @@ void Engine::aca_compress() {
   std::vector<int> ranks(nb);
   HODLR_HIP_CHECK(hipMemcpyAsync(ranks.data(), d_ranks.p, nb * sizeof(int), hipMemcpyDeviceToHost, stream_));
   HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
-  for (int id = 0; id < nn; id++) {
-    nodes_[id].kt = ranks[2 * id];
-    nodes_[id].kb = ranks[2 * id + 1];
-    const int l = nodes_[id].level;
-    K_[l] = std::max(K_[l], std::max(nodes_[id].kt, nodes_[id].kb));
-  }
-  off_[0] = 0;
-  for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
-  ktot_ = off_[nlev_];
+  for (int b = 0; b < nb; b++)
+    if (blk[b].skip != 2) blk_rank[b] = ranks[b];
   if (!pcap.empty()) {  // how the predictor did: misses need the adaptive growth
-    stats_.pred_blocks = nb;
+    stats_.pred_blocks = 0;
     stats_.pred_misses = 0;
     double err = 0.0;
     for (int b = 0; b < nb; b++) {
+      if (blk[b].skip != 0) continue;
+      stats_.pred_blocks++;
       if (ranks[b] > pcap[b]) stats_.pred_misses++;
       err += std::fabs(std::log(std::max(pred_rank_[b], 1.0)) - std::log((double)std::max(ranks[b], 1)));
     }
-    stats_.pred_log_err = err / nb;
+    stats_.pred_log_err = stats_.pred_blocks ? err / stats_.pred_blocks : 0.0;
   }
   // algorithmic bytes of the residual passes: 8 (m+n) k(k+1)/2 per block
   double bytes = 0;
@@ -915,21 +1006,18 @@ void Engine::aca_compress() {
  * (stable).  The rank is the number of sigma_j >= tol (absolute, :529-533);
  * rank 0 stores one zero column (:549-561).  Stored: U Sigma and V, as the
  * reference does. */
-void Engine::svd_compress() {
-  const int nn = (int)nodes_.size();
-  U_.clear();
-  V_.clear();
-  U_.resize(nlev_);
-  V_.resize(nlev_);
-  K_.assign(nlev_, 0);
-  off_.assign(nlev_ + 1, 0);
-  if (nn == 0) {
-    ktot_ = 0;
-    return;
-  }
-  const int nb = 2 * nn;
+void Engine::svd_compress(std::vector<int>& blk_rank) {
+  const int nn_all = (int)nodes_.size();
+  std::vector<int> bid;  // global block index (2 * node + half) of each SVD block
+  for (int id = 0; id < nn_all; id++)
+    if (node_method(nodes_[id]) == 1) {
+      bid.push_back(2 * id);
+      bid.push_back(2 * id + 1);
+    }
+  if (bid.empty()) return;
+  const int nb = (int)bid.size();
   struct Blk {
-    int m, n, mm, nn_, trans, row_u, row_v, level;
+    int m, n, mm, nn_, trans, row_u, row_v, level, min_rank;
     size_t a_off, v_off;
   };
   std::vector<Blk> bl(nb);
@@ -937,11 +1025,13 @@ void Engine::svd_compress() {
   std::vector<AssembleDesc> ad;
   std::vector<CopyDesc> idd;
   int max_e = 0;
-  for (int id = 0; id < nn; id++) {
+  for (int bi = 0; bi < nb; bi++) {
+    const int id = bid[bi] / 2, h = bid[bi] % 2;
     const TreeNodeH& nd = nodes_[id];
     const int st = nd.split - nd.min_i + 1, sb = nd.max_i - nd.split;
-    for (int h = 0; h < 2; h++) {
-      Blk& b = bl[2 * id + h];
+    {
+      Blk& b = bl[bi];
+      b.min_rank = nd.min_rank[h];
       b.m = h == 0 ? st : sb;
       b.n = h == 0 ? sb : st;
       b.trans = b.m < b.n;
@@ -1039,15 +1129,30 @@ void Engine::svd_compress() {
         r = j;
         break;
       }
+    // at least the node's minimum rank (lowRank.cpp:534-535), at most min(m, n)
+    r = std::min(bl[b].nn_, std::max(r, bl[b].min_rank));
     rank[b] = r;
-    K_[bl[b].level] = std::max(K_[bl[b].level], std::max(r, 1));
   }
+  // panel columns each level needs (rank 0 keeps one zero column); panels
+  // that ACA already holds for other blocks of the level grow if needed
+  std::vector<int> need(nlev_, 0);
+  for (int b = 0; b < nb; b++) need[bl[b].level] = std::max(need[bl[b].level], std::max(rank[b], 1));
   for (int l = 0; l < nlev_; l++) {
-    if (K_[l] == 0) continue;
-    U_[l].alloc((size_t)ntot_ * K_[l]);
-    V_[l].alloc((size_t)ntot_ * K_[l]);
-    HODLR_HIP_CHECK(hipMemsetAsync(U_[l].p, 0, U_[l].bytes(), stream_));
-    HODLR_HIP_CHECK(hipMemsetAsync(V_[l].p, 0, V_[l].bytes(), stream_));
+    if (need[l] == 0) continue;
+    const size_t cur = U_[l].n / std::max<size_t>(1, (size_t)ntot_);
+    if ((size_t)need[l] <= cur) continue;
+    DBuf<double> u, v;
+    u.alloc((size_t)ntot_ * need[l]);
+    v.alloc((size_t)ntot_ * need[l]);
+    HODLR_HIP_CHECK(hipMemsetAsync(u.p, 0, u.bytes(), stream_));
+    HODLR_HIP_CHECK(hipMemsetAsync(v.p, 0, v.bytes(), stream_));
+    if (cur > 0) {
+      HODLR_HIP_CHECK(hipMemcpyAsync(u.p, U_[l].p, U_[l].bytes(), hipMemcpyDeviceToDevice, stream_));
+      HODLR_HIP_CHECK(hipMemcpyAsync(v.p, V_[l].p, V_[l].bytes(), hipMemcpyDeviceToDevice, stream_));
+    }
+    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    U_[l] = std::move(u);
+    V_[l] = std::move(v);
   }
   // U Sigma and V into the level panels (rank 0 keeps its zero column)
   std::vector<ColCopyDesc> cc;
@@ -1075,14 +1180,7 @@ void Engine::svd_compress() {
     launch_col_copy(arena_.push(part, stream_), cnt_c, stream_);
   }
   HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
-  for (int id = 0; id < nn; id++) {
-    nodes_[id].kt = std::max(rank[2 * id], 1);
-    nodes_[id].kb = std::max(rank[2 * id + 1], 1);
-  }
-  off_[0] = 0;
-  for (int l = 0; l < nlev_; l++) off_[l + 1] = off_[l] + K_[l];
-  ktot_ = off_[nlev_];
-  stats_.aca_bytes = 0;
+  for (int b = 0; b < nb; b++) blk_rank[bid[b]] = std::max(rank[b], 1);
 }
 
 /* Features of every off-diagonal block (2 per node, node order: top, then

@@ -133,6 +133,19 @@ struct SolveGraph {
   ~SolveGraph() { release(); }
 };
 
+/* A node of a user-defined partition (user_IndexTree::node,
+ * include/user_IndexTree.hpp:10-24), children by index into the node list
+ * (-1: none).  A node with split -1 and at most leaf rows is a leaf; a node
+ * without children gets default subtrees below its split
+ * (HODLR_Tree.cpp:207-321). */
+struct UserNode {
+  int split = -1;
+  int min_rank[2] = {-1, -1};  // top, bottom off-diagonal block
+  int max_rank[2] = {-1, -1};
+  int lr = -1;                 // 0 partial-pivot ACA, 1 SVD, -1 the handle's method
+  int left = -1, right = -1;
+};
+
 struct InstanceSpec {
   int n = 0;
   int leaf = 30;
@@ -140,6 +153,7 @@ struct InstanceSpec {
   HodlrKernelParams kp{};
   std::vector<double> xs, ys, zs;  // coordinates (kernel)
   std::vector<double> dense;       // n x n (dense)
+  std::vector<UserNode> tree;      // user partition (root first); empty: the default tree
 };
 
 struct TreeNodeH {
@@ -147,6 +161,9 @@ struct TreeNodeH {
   int min_i, split, max_i;  // instance-local
   int row0;                 // global row of min_i
   int kt = 0, kb = 0;
+  int lr = -1;                   // low-rank method of this node (-1: the handle's)
+  int min_rank[2] = {-1, -1};    // rank bounds of the top and bottom blocks
+  int max_rank[2] = {-1, -1};
 };
 
 struct LeafH {
@@ -214,6 +231,14 @@ class Engine {
   const std::vector<TreeNodeH>& nodes() const { return nodes_; }
   const std::vector<LeafH>& leaves() const { return leaves_; }
   void get_factor(int idx, int which, double* host_out);
+  /* Entries of rows [i0, i0+m) and columns [j0, j0+n): which 0 the source
+   * matrix (dense copy or kernel), 1 the compressed operator A_H; output
+   * column-major with leading dimension ldo (device or host pointer). */
+  void block_device(int which, int i0, int j0, int m, int n, double* dOut, int ldo);
+  void block_host(int which, int i0, int j0, int m, int n, double* out, int ldo);
+  /* The root's top (side 0) or bottom (side 1) diagonal block as a matrix
+   * of its own, tree part and compressed factors kept. */
+  std::unique_ptr<Engine> diagonal_block(int side);
   struct Stats {
     int aca_iterations = 0;
     // factor_time: from the factorization's first kernel to its last one on
@@ -256,8 +281,11 @@ class Engine {
   void build_trees();
   void upload_sources();
   void reset_attributes();
-  void aca_compress();
-  void svd_compress();
+  // each compresses the blocks whose node uses its method; ranks per block
+  // (2 per node: top, bottom) into blk_rank
+  void aca_compress(std::vector<int>& blk_rank);
+  void svd_compress(std::vector<int>& blk_rank);
+  int node_method(const TreeNodeH& nd) const { return nd.lr >= 0 ? nd.lr : lr_method_; }
   void block_features_device(double tol, DBuf<float>& out);
   std::vector<int> predicted_capacities();
   void assemble_leaves();

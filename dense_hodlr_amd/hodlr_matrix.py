@@ -15,10 +15,15 @@ import numpy as np
 
 from . import _lib
 from ._lib import HodlrError, HodlrNotConverged, check, lib
+from .user_tree import as_tree_array, tree_of
 
 
 def _dptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _iptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
 def _as_f64_2d(B, n):
@@ -41,7 +46,10 @@ class HODLR_Matrix:
     """
 
     def __init__(self, matrix=None, sizeThreshold: int = 30, LR_Method: str = "partialPiv_ACA",
-                 *, _handle=None, _n=None):
+                 *, indexTree=None, _handle=None, _n=None):
+        """indexTree: a user partition (user_IndexTree, or its node array), the
+        constructor HODLR_Matrix(MatrixXd&, int, user_IndexTree&)
+        (HODLR_Matrix.hpp:85)."""
         self._h = None
         if _handle is not None:
             self._h = _handle
@@ -54,14 +62,27 @@ class HODLR_Matrix:
         if LR_Method not in _lib.LR_METHODS:
             raise ValueError(f"Invalid low-rank approximation scheme ({LR_Method})")
         h = C.c_void_p()
-        check(lib().hodlr_create_dense(C.byref(h), A.shape[0], _dptr(A), A.shape[0],
-                                       int(sizeThreshold), _lib.LR_METHODS[LR_Method]))
+        if indexTree is None:
+            check(lib().hodlr_create_dense(C.byref(h), A.shape[0], _dptr(A), A.shape[0],
+                                           int(sizeThreshold), _lib.LR_METHODS[LR_Method]))
+        else:
+            T = as_tree_array(indexTree)
+            check(lib().hodlr_create_dense_tree(C.byref(h), A.shape[0], _dptr(A), A.shape[0],
+                                                int(sizeThreshold), _iptr(T), len(T)))
+            self._h = h
+            self._n = A.shape[0]
+            if LR_Method != "partialPiv_ACA":
+                self.set_LRMethod(LR_Method)
+            return
         self._h = h
         self._n = A.shape[0]
 
     @classmethod
     def from_kernel(cls, points, kernel: str = "gaussian", length_scale: float = 1.0,
-                    diag: float = 0.0, diag_mode: str = "add", sizeThreshold: int = 30):
+                    diag: float = 0.0, diag_mode: str = "add", sizeThreshold: int = 30,
+                    indexTree=None):
+        """indexTree: a user partition, the constructor with a
+        user_IndexTree& (HODLR_Matrix.hpp:184)."""
         pts = np.ascontiguousarray(np.asarray(points, dtype=np.float64))
         if pts.ndim == 1:
             pts = pts.reshape(-1, 1)
@@ -69,8 +90,26 @@ class HODLR_Matrix:
         kd = _lib.KernelDesc(_lib.KERNELS[kernel], dim, _dptr(pts), float(length_scale),
                              float(diag), {"add": 0, "replace": 1}[diag_mode])
         h = C.c_void_p()
-        check(lib().hodlr_create_kernel(C.byref(h), n, C.byref(kd), int(sizeThreshold)))
+        if indexTree is None:
+            check(lib().hodlr_create_kernel(C.byref(h), n, C.byref(kd), int(sizeThreshold)))
+        else:
+            T = as_tree_array(indexTree)
+            check(lib().hodlr_create_kernel_tree(C.byref(h), n, C.byref(kd), int(sizeThreshold),
+                                                 _iptr(T), len(T)))
         return cls(_handle=h, _n=n)
+
+    @classmethod
+    def from_binary_file(cls, path, sizeThreshold: int = 30, LR_Method: str = "partialPiv_ACA"):
+        """HODLR_Matrix(readBinaryIntoMatrixXd(path), sizeThreshold, LR_Method), the
+        reference drivers' way to build from a file (helperFunctions.cpp:370-371),
+        with the file streamed straight into the handle's copy."""
+        import os
+        if LR_Method not in _lib.LR_METHODS:
+            raise ValueError(f"Invalid low-rank approximation scheme ({LR_Method})")
+        h = C.c_void_p()
+        check(lib().hodlr_create_dense_from_file(C.byref(h), os.fsencode(path), int(sizeThreshold),
+                                                 _lib.LR_METHODS[LR_Method]))
+        return cls(_handle=h, _n=lib().hodlr_rows(h))
 
     @classmethod
     def from_callback(cls, n: int, fn, data=None, sizeThreshold: int = 30):
@@ -179,6 +218,56 @@ class HODLR_Matrix:
         check(lib().hodlr_residual(self._h, _dptr(B), self._n, B.shape[1], _dptr(X), self._n,
                                    _dptr(R), self._n), self._h)
         return R
+
+    # ---- tree surgery (HODLR_Matrix.hpp:256-261) ---------------------------
+    def _diag(self, side):
+        h = C.c_void_p()
+        check(lib().hodlr_diag_block(self._h, int(side), C.byref(h)), self._h)
+        return HODLR_Matrix(_handle=h, _n=lib().hodlr_rows(h))
+
+    def topDiag(self):
+        """The root's top-left diagonal block as a HODLR matrix (tree part and
+        factors kept; this matrix is unchanged)."""
+        return self._diag(0)
+
+    def bottDiag(self):
+        return self._diag(1)
+
+    def splitAtTop(self):
+        """splitAtTop(self, topHODLR, bottHODLR): (top, bottom); unlike the
+        reference, this matrix keeps its tree."""
+        t, b = C.c_void_p(), C.c_void_p()
+        check(lib().hodlr_split_at_top(self._h, C.byref(t), C.byref(b)), self._h)
+        return (HODLR_Matrix(_handle=t, _n=lib().hodlr_rows(t)),
+                HODLR_Matrix(_handle=b, _n=lib().hodlr_rows(b)))
+
+    def user_tree(self):
+        """This matrix's partition as a user tree node array (every node
+        explicit, no rank bounds)."""
+        return tree_of(self.nodes(), self._n)
+
+    # ---- entries (HODLR_Matrix.hpp:252-254) -------------------------------
+    def _block(self, which, min_i, min_j, numRows, numCols):
+        out = np.zeros((numRows, numCols), order="F")
+        check(lib().hodlr_block(self._h, which, int(min_i), int(min_j), int(numRows),
+                                int(numCols), _dptr(out), max(1, int(numRows))), self._h)
+        return out
+
+    def block(self, min_i: int, min_j: int, numRows: int, numCols: int):
+        """Entries of the matrix the handle was built from (the reference's
+        block() when it holds the matrix data)."""
+        return self._block(0, min_i, min_j, numRows, numCols)
+
+    def row(self, row: int):
+        return self.block(row, 0, 1, self._n)
+
+    def col(self, col: int):
+        return self.block(0, col, self._n, 1)
+
+    def block_compressed(self, min_i: int, min_j: int, numRows: int, numCols: int):
+        """Entries of the compressed operator A_H (leaves plus U V^T: the
+        reference's fill_Block, HODLR_Matrix.cpp:1424-1480)."""
+        return self._block(1, min_i, min_j, numRows, numCols)
 
     # ---- accessors -------------------------------------------------------
     def rows(self) -> int:

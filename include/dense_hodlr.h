@@ -132,6 +132,24 @@ typedef struct hodlr_stats {
                                 stream, where a following solve starts */
 } hodlr_stats;
 
+/* One node of a user-defined partition: user_IndexTree::node
+ * (include/user_IndexTree.hpp:10-24) with its children as indices into the
+ * node array (node 0 is the root).  split is the last row of the node's top
+ * (left) part; a node with split -1 and at most leaf rows is a leaf; a node
+ * without children (left = right = -1) gets default subtrees below its
+ * split, as the reference's createFromUsrTree does (HODLR_Tree.cpp:207-321);
+ * a node must have both children or none.  Rank bounds of the node's top
+ * and bottom off-diagonal blocks (-1: none; a maximum of 0 stores a zero
+ * block) and its method (HODLR_LR_ACA, HODLR_LR_SVD, or -1 for the handle's
+ * method) correspond to topOffDiag_minRank .. LR_Method. */
+typedef struct hodlr_tree_node {
+  int split;
+  int top_min_rank, bottom_min_rank;
+  int top_max_rank, bottom_max_rank;
+  int lr_method;
+  int left, right;
+} hodlr_tree_node;
+
 /* ---- construction ------------------------------------------------------ */
 
 /* HODLR_Matrix(int nRows, int nCols, double(*)(int,int,void*), void*, int leaf)
@@ -149,6 +167,53 @@ int hodlr_create_kernel_callback(hodlr_handle* h, int n, double (*fn)(int, int, 
  * (include/HODLR_Matrix.hpp:76, src/HODLR_Matrix.cpp:89-98): copies A. */
 int hodlr_create_dense(hodlr_handle* h, int n, const double* A, int lda, int leaf,
                        int lr_method);
+
+/* HODLR_Matrix(readBinaryIntoMatrixXd(path), leaf, LR) -- how the
+ * reference's drivers build from a file (src/helperFunctions.cpp:370-371):
+ * the dense constructor on a matrix read from the binary format of
+ * hodlr_read_binary_matrix, streamed into the handle's copy. */
+int hodlr_create_dense_from_file(hodlr_handle* h, const char* path, int leaf, int lr_method);
+
+/* The kernel and dense constructors with a user partition:
+ * HODLR_Matrix(int, int, double(*)(int,int,void*), void*, int,
+ * user_IndexTree&) (include/HODLR_Matrix.hpp:184) and
+ * HODLR_Matrix(Eigen::MatrixXd&, int, user_IndexTree&) (:85,
+ * HODLR_Matrix.cpp:100-108).  count == 0 gives the default tree. */
+int hodlr_create_kernel_tree(hodlr_handle* h, int n, const hodlr_kernel_desc* kernel, int leaf,
+                             const hodlr_tree_node* nodes, int count);
+int hodlr_create_dense_tree(hodlr_handle* h, int n, const double* A, int lda, int leaf,
+                            const hodlr_tree_node* nodes, int count);
+
+/* kd order of npoints points of dim coordinates (point-major), the
+ * reference's get_KDTree_Sorted(locations, 0) (rigidBody/MobFunctions.h:
+ * 334-369: merge sort by coordinate depth % dim, halves of n/2 and the rest,
+ * equal keys taking the right list first): perm[i] is the point at position
+ * i.  Host only. */
+int hodlr_kd_sort(const double* points, int npoints, int dim, int* perm);
+/* The same order and its partition, get_KDTree_Sorted(locations,
+ * pointsPerSphere, rankUpperBound, LR_Method) (MobFunctions.h:371-430): one
+ * node per segment (2 npoints - 1 of them, root first) with split =
+ * unknowns_per_point * (first point of the right part) - 1, both blocks'
+ * maximum rank max_rank, method lr_method; single points are leaves.  nodes
+ * holds 2 npoints - 1 entries; *count receives the number written.  Host
+ * only. */
+int hodlr_kd_tree(const double* points, int npoints, int dim, int unknowns_per_point, int max_rank,
+                  int lr_method, int* perm, hodlr_tree_node* nodes, int* count);
+
+/* ---- tree surgery (HODLR_Matrix.hpp:256-261) ---------------------------- */
+
+/* topDiag() (side 0) and bottDiag() (side 1) (HODLR_Matrix.cpp:1551-1603):
+ * a new handle for the root's top-left or bottom-right diagonal block, with
+ * that part of the tree and of the compressed factors (compressing h first
+ * if needed; nothing is recompressed), the block's source data (the dense
+ * block, or the kernel on its points) and h's settings.  h is unchanged.
+ * A root that is a leaf returns HODLR_ERR_STATE (the reference returns a
+ * copy or an empty matrix with a warning); a batch HODLR_ERR_ARG. */
+int hodlr_diag_block(hodlr_handle h, int side, hodlr_handle* out);
+/* splitAtTop(self, topHODLR, bottHODLR) (HODLR_Matrix.hpp:261,
+ * HODLR_Matrix.cpp:1507-1549): both diagonal blocks at once.  Unlike the
+ * reference, which deletes self's tree, h stays usable. */
+int hodlr_split_at_top(hodlr_handle h, hodlr_handle* top, hodlr_handle* bottom);
 
 /* Releases all resources (~HODLR_Matrix). */
 int hodlr_destroy(hodlr_handle h);
@@ -248,6 +313,24 @@ int hodlr_matvec_device_noalloc(hodlr_handle h, const double* dX, int ldx, int n
 int hodlr_residual(hodlr_handle h, const double* B, int ldb, int nrhs, const double* X, int ldx,
                    double* R, int ldr);
 
+/* ---- matrix files (src/matrixIO.cpp) ------------------------------------ */
+
+/* Binary format of saveMatrixXdToBinary / readBinaryIntoMatrixXd
+ * (include/matrixIO.hpp:42, :51; src/matrixIO.cpp:165-208): int32 rows, int32
+ * cols, then rows x cols doubles in row-major order, native byte order.  With
+ * out == NULL only rows and cols are returned; else out (column-major,
+ * ld >= rows) receives the matrix.  Failures (file missing, short or of
+ * another size) return HODLR_ERR_ARG with the message in
+ * hodlr_last_error(NULL); the reference prints "Error! File ... could not be
+ * opened" and exits. */
+int hodlr_read_binary_matrix(const char* path, int* rows, int* cols, double* out, int ld);
+int hodlr_write_binary_matrix(const char* path, int rows, int cols, const double* A, int lda);
+/* Text format of readTxtIntoMatrix (matrixIO.hpp:20, matrixIO.cpp:11-50): a
+ * line "rows cols", then lines "i j value" (0-based); entries not listed are
+ * zero (the reference leaves them uninitialized).  Same calling convention
+ * as hodlr_read_binary_matrix. */
+int hodlr_read_text_matrix(const char* path, int* rows, int* cols, double* out, int ld);
+
 /* ---- batched instances (no reference counterpart) ----------------------- */
 
 /* count independent kernel systems in one handle, for many small or uneven
@@ -327,6 +410,20 @@ int hodlr_get_nodes(hodlr_handle h, int* level, int* min_i, int* split, int* max
  * U, 3 bottom V (topOffDiagU/V, bottOffDiagU/V, HODLR_Tree.hpp:41-44); out is
  * rows x rank column-major.  Returns the rank or a negative status. */
 int hodlr_get_factor(hodlr_handle h, int idx, int which, double* out);
+/* Entry access: block(min_i, min_j, numRows, numCols), row(i) = block(i, 0,
+ * 1, n) and col(j) = block(0, j, n, 1) (HODLR_Matrix.hpp:252-254,
+ * HODLR_Matrix.cpp:1396-1480).  which = HODLR_BLOCK_SOURCE returns entries of
+ * the matrix the handle was built from (its dense copy, or the kernel
+ * evaluated on the device), as the reference's block() does when it holds
+ * that data; HODLR_BLOCK_COMPRESSED returns the compressed operator A_H
+ * (leaves plus U V^T, the reference's fill_Block), compressing first if
+ * needed.  out is m x n column-major with leading dimension ldo >= m: host
+ * memory (hodlr_block) or device memory (hodlr_block_device).  For a batch
+ * the matrix is the block-diagonal system of the instances. */
+enum { HODLR_BLOCK_SOURCE = 0, HODLR_BLOCK_COMPRESSED = 1 };
+int hodlr_block(hodlr_handle h, int which, int i0, int j0, int m, int n, double* out, int ldo);
+int hodlr_block_device(hodlr_handle h, int which, int i0, int j0, int m, int n, double* dout,
+                       int ldo);
 /* Message of the last failure on this handle (or of the last failed create
  * when h is NULL). */
 const char* hodlr_last_error(hodlr_handle h);

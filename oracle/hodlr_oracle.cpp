@@ -502,6 +502,8 @@ struct Node {
   int level = 0, min_i = 0, max_i = 0, split = -1;
   bool leaf = false;
   int kt = 0, kb = 0;
+  int lr = -1;                               // LR_Method of the node (-1: the matrix's)
+  int minR[2] = {-1, -1}, maxR[2] = {-1, -1};  // topOffDiag_/bottOffDiag_ min/maxRank
   Mat tU, tV, bU, bV;  // top: rows [min,split] x cols [split+1,max]
   Mat tU_SM, bU_SM;
   Mat leafMat;
@@ -580,6 +582,71 @@ struct Hodlr {
       build(root);
     }
   }
+  /* userTree_To_HODLRTree (HODLR_Tree.cpp:207-321); user nodes are 8 ints:
+   * split, top/bottom min rank, top/bottom max rank, method, left, right */
+  void fromUser(const int* un, int ui, int level, int lo, int hi, Node* nd) {
+    nd->level = level;
+    nd->min_i = lo;
+    nd->max_i = hi;
+    if (level > numLevels - 1) numLevels = level + 1;
+    const int* u = un + 8 * ui;
+    const int size = hi - lo + 1;
+    if (size <= leaf && u[0] == -1) {  // :212-231
+      nd->leaf = true;
+      leaves.push_back(nd);
+      return;
+    }
+    nd->split = u[0];  // :233-246
+    nd->minR[0] = u[1];
+    nd->minR[1] = u[2];
+    nd->maxR[0] = u[3];
+    nd->maxR[1] = u[4];
+    nd->lr = u[5];
+    addLevel(nd);
+    Node* L = newNode();
+    Node* R = newNode();
+    nd->left = L;
+    nd->right = R;
+    if (u[6] == -1 && u[7] == -1) {  // default children (:259-315)
+      const int ls = u[0] - lo + 1, rs = hi - u[0];
+      L->level = level + 1;
+      L->min_i = lo;
+      L->max_i = u[0];
+      if (ls <= leaf) {
+        L->leaf = true;
+        leaves.push_back(L);
+        if (L->level > numLevels - 1) numLevels = L->level + 1;
+      } else {
+        L->split = lo + ls / 2 - 1;
+        addLevel(L);
+        build(L);
+      }
+      R->level = level + 1;
+      R->min_i = u[0] + 1;
+      R->max_i = hi;
+      if (rs <= leaf) {
+        R->leaf = true;
+        leaves.push_back(R);
+        if (R->level > numLevels - 1) numLevels = R->level + 1;
+      } else {
+        R->split = u[0] + rs / 2;  // (the reference's rule here, not max - rs/2)
+        addLevel(R);
+        build(R);
+      }
+    } else {  // :316-319
+      fromUser(un, u[6], level + 1, lo, u[0], L);
+      fromUser(un, u[7], level + 1, u[0] + 1, hi, R);
+    }
+  }
+  void createFromUser(const int* un, int count) {  // createFromUsrTree (:323-331)
+    if (count == 0) {
+      createTree();
+      return;
+    }
+    root = newNode();
+    numLevels = 1;
+    fromUser(un, 0, 0, 0, n - 1, root);
+  }
   void resetAttributes() {  // HODLR_Matrix.cpp:1232-1246
     compressed = smFactored = luFactored = false;
     tCompress = tSmFactor = tSmSolve = tLuFactor = tLuSolve = 0;
@@ -595,16 +662,17 @@ struct Hodlr {
     }
     int mT = r->split - r->min_i + 1, nT = r->max_i - r->split;
     int mB = r->max_i - r->split, nB = r->split - r->min_i + 1;
-    if (method == 0) {
-      aca_partial_pivot(src, r->min_i, r->split + 1, mT, nT, tol, -1, -1, minPivot, r->tU,
-                        r->tV, r->kt, nullptr);
-      aca_partial_pivot(src, r->split + 1, r->min_i, mB, nB, tol, -1, -1, minPivot, r->bU,
-                        r->bV, r->kb, nullptr);
+    const int m = r->lr >= 0 ? r->lr : method;  // the node's LR_Method (:330-380)
+    if (m == 0) {
+      aca_partial_pivot(src, r->min_i, r->split + 1, mT, nT, tol, r->minR[0], r->maxR[0],
+                        minPivot, r->tU, r->tV, r->kt, nullptr);
+      aca_partial_pivot(src, r->split + 1, r->min_i, mB, nB, tol, r->minR[1], r->maxR[1],
+                        minPivot, r->bU, r->bV, r->kb, nullptr);
     } else {
-      r->kt = svd_lowrank(src.block(r->min_i, r->split + 1, mT, nT), tol, -1, r->tU, r->tV,
-                          nullptr);
-      r->kb = svd_lowrank(src.block(r->split + 1, r->min_i, mB, nB), tol, -1, r->bU, r->bV,
-                          nullptr);
+      r->kt = svd_lowrank(src.block(r->min_i, r->split + 1, mT, nT), tol, r->minR[0], r->tU,
+                          r->tV, nullptr);
+      r->kb = svd_lowrank(src.block(r->split + 1, r->min_i, mB, nB), tol, r->minR[1], r->bU,
+                          r->bV, nullptr);
     }
     store(r->left);
     store(r->right);
@@ -921,6 +989,30 @@ void* orc_create_kernel(int n, int family, int dim, const double* pts, double el
   return h;
 }
 
+void* orc_create_kernel_tree(int n, int family, int dim, const double* pts, double ell, double diag,
+                             int diag_mode, int leaf, const int* nodes, int count) {
+  Hodlr* h = new Hodlr();
+  fill_kernel(h->src, n, family, dim, pts, ell, diag, diag_mode);
+  h->n = n;
+  h->leaf = leaf;
+  h->createFromUser(nodes, count);
+  return h;
+}
+
+void* orc_create_dense_tree(int n, const double* A, int lda, int leaf, const int* nodes,
+                            int count) {
+  Hodlr* h = new Hodlr();
+  h->src.dense = true;
+  h->src.n = n;
+  h->src.A.resize((size_t)n * n);
+  for (int j = 0; j < n; j++)
+    for (int i = 0; i < n; i++) h->src.A[(size_t)i + (size_t)j * n] = A[(size_t)i + (size_t)j * lda];
+  h->n = n;
+  h->leaf = leaf;
+  h->createFromUser(nodes, count);
+  return h;
+}
+
 void* orc_create_dense(int n, const double* A, int lda, int leaf) {
   Hodlr* h = new Hodlr();
   h->src.dense = true;
@@ -965,10 +1057,11 @@ void orc_set_min_pivot(void* p, double mp) {
 
 void orc_set_lr_method(void* p, int m) {
   Hodlr* h = (Hodlr*)p;
-  if (h->method != m) {
+  if (h->method != m) {  // HODLR_Matrix.cpp:1277-1286, HODLR_Tree.cpp:377-390
     h->method = m;
     h->resetAttributes();
     for (auto& nd : h->pool) {
+      nd->lr = -1;
       nd->tU_SM = Mat();
       nd->bU_SM = Mat();
       nd->pI = PerturbI();

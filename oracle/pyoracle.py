@@ -43,6 +43,10 @@ def lib():
         L.orc_create_kernel.argtypes = [_i, _i, _i, _dp, _d, _d, _i, _i]
         L.orc_create_dense.restype = _p
         L.orc_create_dense.argtypes = [_i, _dp, _i, _i]
+        L.orc_create_kernel_tree.restype = _p
+        L.orc_create_kernel_tree.argtypes = [_i, _i, _i, _dp, _d, _d, _i, _i, _ip, _i]
+        L.orc_create_dense_tree.restype = _p
+        L.orc_create_dense_tree.argtypes = [_i, _dp, _i, _i, _ip, _i]
         L.orc_destroy.argtypes = [_p]
         L.orc_set_tolerance.argtypes = [_p, _d]
         L.orc_set_min_pivot.argtypes = [_p, _d]
@@ -145,16 +149,27 @@ class OracleHODLR:
     constructors (include/HODLR_Matrix.hpp:76, :162)."""
 
     def __init__(self, n, leaf=30, points=None, family=None, length_scale=1.0, diag=0.0,
-                 diag_mode=0, dense=None):
+                 diag_mode=0, dense=None, tree=None):
+        """tree: user partition, rows of 8 ints (split, top/bottom min rank,
+        top/bottom max rank, method, left, right), root first."""
         L = lib()
         self.n = n
+        T = None if tree is None else np.ascontiguousarray(np.asarray(tree, dtype=np.int32).reshape(-1, 8))
         if dense is not None:
             A = np.asfortranarray(dense, dtype=np.float64)
-            self.h = L.orc_create_dense(n, _ptr(A), n, leaf)
+            if T is None:
+                self.h = L.orc_create_dense(n, _ptr(A), n, leaf)
+            else:
+                self.h = L.orc_create_dense_tree(n, _ptr(A), n, leaf, _iptr(T), len(T))
         else:
             self._pts = np.ascontiguousarray(points, dtype=np.float64)
-            self.h = L.orc_create_kernel(n, FAMILIES[family], self._pts.shape[1], _ptr(self._pts),
-                                         length_scale, diag, diag_mode, leaf)
+            if T is None:
+                self.h = L.orc_create_kernel(n, FAMILIES[family], self._pts.shape[1],
+                                             _ptr(self._pts), length_scale, diag, diag_mode, leaf)
+            else:
+                self.h = L.orc_create_kernel_tree(n, FAMILIES[family], self._pts.shape[1],
+                                                  _ptr(self._pts), length_scale, diag, diag_mode,
+                                                  leaf, _iptr(T), len(T))
 
     def __del__(self):
         if getattr(self, "h", None):

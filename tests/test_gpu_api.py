@@ -126,3 +126,25 @@ def test_factor_time_includes_root_lu():
           flush=True)
     assert st["factor_time"] > 0 and st["factor_stream_time"] > 0
     assert st["factor_time"] >= st["factor_stream_time"]
+
+
+def test_dense_constructor_from_binary_file(tmp_path, oracle):
+    """HODLR_Matrix(readBinaryIntoMatrixXd(path), ...) as the reference's
+    drivers build it (helperFunctions.cpp:370-371): the handle made from the
+    file equals the one made from the array it was written from."""
+    from dense_hodlr_amd import saveMatrixXdToBinary
+    _case("dense constructor from a binary file")
+    n, leaf = 1200, 50
+    A = I.exact_hodlr(n, leaf, 6, seed=31)
+    path = tmp_path / "A.bin"
+    saveMatrixXdToBinary(A, path)
+    b = I.rhs_uniform(n, 2, 32)
+    g1 = HODLR_Matrix(A, leaf, "partialPiv_ACA")
+    g2 = HODLR_Matrix.from_binary_file(path, leaf, "partialPiv_ACA")
+    for g in (g1, g2):
+        g.set_LRTolerance(1e-10)
+    x1, x2 = g1.recSM_Solve(b), g2.recSM_Solve(b)
+    assert np.array_equal(g1.nodes(), g2.nodes()) and np.array_equal(x1, x2)
+    o = oracle.OracleHODLR(n, leaf=leaf, dense=A)
+    o.set_LRTolerance(1e-10)
+    assert rel(x2, o.recSM_Solve(b)) < 1e-10
