@@ -90,6 +90,7 @@ class Stats(C.Structure):
         ("tuned", C.c_int),
         ("tune_gemm_mfma", C.c_int),
         ("factor_stream_time", C.c_double),
+        ("tune_leaf_blocked", C.c_int),
     ]
 
 
@@ -148,7 +149,8 @@ def lib():
         "hodlr_set_profiling": (C.c_int, [h, C.c_int]),
         "hodlr_set_graphs": (C.c_int, [h, C.c_int]),
         "hodlr_autotune": (C.c_int, [h]),
-        "hodlr_set_launch_shapes": (C.c_int, [h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "hodlr_set_launch_shapes": (C.c_int, [h, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
+                                              C.c_int]),
         "hodlr_compress": (C.c_int, [h]),
         "hodlr_factorize": (C.c_int, [h, C.c_int]),
         "hodlr_solve_sm": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int]),

@@ -114,6 +114,8 @@ void Engine::set_tuning(const Tuning& t) {
   if (t.gemm_target < 1) throw std::invalid_argument("split-K target must be positive");
   if (t.lu_nb < 1 || t.lu_nb > LU_NB) throw std::invalid_argument("LU panel width must be 1..64");
   if (t.gemm_mfma != 0 && t.gemm_mfma != 1) throw std::invalid_argument("mfma must be 0 or 1");
+  if (t.leaf_blk != 0 && t.leaf_blk != 1)
+    throw std::invalid_argument("blocked leaf solves must be 0 or 1");
   tune_ = t;
   tune_.tuned = false;
   reset_attributes();
@@ -165,6 +167,15 @@ void Engine::autotune() {
         fb = t;
       }
     }
+  {  // the leaf solves of the factorization: one launch or blocked
+    Tuning t = fb;
+    t.leaf_blk = 1 - fb.leaf_blk;
+    const double v = std::min(time_factor(t), time_factor(t));
+    if (v < bt) {
+      bt = v;
+      fb = t;
+    }
+  }
   fb.tuned = true;
   tune_ = fb;
   reset_attributes();
@@ -467,7 +478,7 @@ long long Engine::device_bytes() const {
   long long b = 0;
   for (auto& x : U_) b += x.bytes();
   for (auto& x : V_) b += x.bytes();
-  b += Lorig_.bytes() + Lfac_.bytes() + Linv_.bytes() + Lpiv_.bytes();
+  b += Lorig_.bytes() + Lfac_.bytes() + Lpiv_.bytes();
   for (auto& f : fac_) b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
   b += ws_.bytes() + tmp_.bytes() + tmp2_.bytes() + xw_.bytes();
   for (auto& f : mfac_) {
@@ -632,11 +643,11 @@ void Engine::lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& aren
 /* B <- A^-1 B for a batch of LU-factored matrices: row interchanges, then
  * blocked forward (unit L) and backward (U) substitution, each block a small
  * triangular solve followed by a grouped GEMM update of the remaining rows. */
-void Engine::lu_solve_batch(std::vector<SolveDesc>& descs) {
+void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, bool blocked) {
   if (descs.empty()) return;
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
-  if (maxn <= solve_small_max_) {  // every system in one launch
+  if (maxn <= solve_small_max_ && !blocked) {  // every system in one launch
     const bool wave = maxn <= solve_wave_max_;
     int nwg = 0;
     for (auto& d : descs) {
@@ -1272,34 +1283,42 @@ void Engine::assemble_leaves() {
 /* ------------------------------------------------------------------ */
 /* factorization                                                       */
 /* ------------------------------------------------------------------ */
+/* The leaves' LU factors (PartialPivLU, HODLR_Matrix.cpp:541, :552). */
 void Engine::factor_leaves() {
   if (leaves_factored_) return;
   Lfac_.alloc(leaf_elems_);
-  Linv_.alloc(leaf_elems_);
   Lpiv_.alloc(leaf_piv_elems_);
   HODLR_HIP_CHECK(hipMemcpyAsync(Lfac_.p, Lorig_.p, leaf_elems_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   std::vector<LuDesc> ld;
-  std::vector<CopyDesc> id;
+  for (auto& lf : leaves_) ld.push_back({Lfac_.p + lf.mat_off, Lpiv_.p + lf.piv_off, lf.s, lf.s});
+  lu_batch(ld);
+  leaves_factored_ = true;
+}
+
+/* B[rows of leaf] <- A_leaf^-1 B[rows of leaf] for the given leaves, by
+ * their LU factors (the reference's leafLU.solve, HODLR_Matrix.cpp:598-601,
+ * :569-572): B points at global row 0 (ld ldb) unless compact, where the
+ * leaves' blocks follow each other (s x ncols each, ld s). */
+void Engine::leaf_solves(const std::vector<int>& leaf_ids, double* B, int ldb, int ncols,
+                         bool compact) {
+  if (leaf_ids.empty() || ncols <= 0) return;
   std::vector<SolveDesc> sd;
-  int maxs = 1;
-  for (auto& lf : leaves_) {
-    ld.push_back({Lfac_.p + lf.mat_off, Lpiv_.p + lf.piv_off, lf.s, lf.s});
-    id.push_back({nullptr, Linv_.p + lf.mat_off, lf.s, lf.s, 0, lf.s});
+  size_t o = 0;
+  for (int li : leaf_ids) {
+    const LeafH& lf = leaves_[li];
     SolveDesc s{};
     s.LU = Lfac_.p + lf.mat_off;
     s.piv = Lpiv_.p + lf.piv_off;
-    s.B = Linv_.p + lf.mat_off;
+    s.B = compact ? B + o : B + lf.row0;
     s.n = lf.s;
     s.ld_lu = lf.s;
-    s.ldb = lf.s;
-    s.ncols = lf.s;
+    s.ldb = compact ? lf.s : ldb;
+    s.ncols = ncols;
     sd.push_back(s);
-    maxs = std::max(maxs, lf.s);
+    o += (size_t)lf.s * ncols;
   }
-  lu_batch(ld);
-  launch_set_identity(arena_.push(id, stream_), (int)id.size(), stream_);
-  lu_solve_batch(sd);
-  leaves_factored_ = true;
+  // many columns (the factorization's leaf step): blocked if the tuning says so
+  lu_solve_batch(sd, tune_.leaf_blk != 0 && ncols >= 128);
 }
 
 /* recSM_Factorize (HODLR_Matrix.cpp:520-592) and recLU_Factorize (:612-692),
@@ -1326,23 +1345,11 @@ void Engine::factorize_fp64(int which) {
   factor_leaves();
   const size_t N = (size_t)ntot_;
   F.W.alloc(std::max<size_t>(1, N * ktot_));
-  // With every leaf on one level (the default trees of power-of-two sizes),
-  // the leaf step reads the U panels and writes W directly: every row of W
-  // is a leaf row there, and its columns of that level and deeper are zero
-  // padding in U as in W's other columns of those rows.  Otherwise W starts
-  // as a copy of the panels.
-  int leaf_levels = 0, leaf_level = -1;
+  // W starts as a copy of the panels (the reference's U_SM = U, :561-565)
   for (int l = 0; l < nlev_; l++)
-    if (!level_leaves_[l].empty()) {
-      leaf_levels++;
-      leaf_level = l;
-    }
-  const bool direct = leaf_levels == 1 && off_[leaf_level] == ktot_;
-  if (!direct)
-    for (int l = 0; l < nlev_; l++)
-      if (K_[l] > 0)
-        HODLR_HIP_CHECK(hipMemcpyAsync(F.W.p + N * off_[l], U_[l].p, N * K_[l] * sizeof(double),
-                                       hipMemcpyDeviceToDevice, stream_));
+    if (K_[l] > 0)
+      HODLR_HIP_CHECK(hipMemcpyAsync(F.W.p + N * off_[l], U_[l].p, N * K_[l] * sizeof(double),
+                                     hipMemcpyDeviceToDevice, stream_));
   const int nn = (int)nodes_.size();
   F.c_off.assign(nn, 0);
   F.p_off.assign(nn, 0);
@@ -1360,34 +1367,10 @@ void Engine::factorize_fp64(int which) {
   const int ld = ntot_;
   for (int d = nlev_ - 1; d >= 0; d--) {
     const int offd = off_[d];
-    // leaves of level d: W[rows, 0:off_d] <- A_leaf^-1 W[rows, 0:off_d]
-    if (offd > 0 && !level_leaves_[d].empty() && direct) {
-      std::vector<GemmDesc> g;
-      for (int li : level_leaves_[d]) {
-        const LeafH& lf = leaves_[li];
-        for (int l = 0; l < d; l++)
-          if (K_[l] > 0)
-            g.push_back(gemm_desc(lf.s, K_[l], lf.s, 0, Linv_.p + lf.mat_off, lf.s, U_[l].p + lf.row0,
-                                  ld, W + lf.row0 + N * off_[l], ld, 1.0, 0.0));
-      }
-      run_gemm(g);
-    } else if (offd > 0 && !level_leaves_[d].empty()) {
-      size_t need = 0;
-      for (int li : level_leaves_[d]) need += (size_t)leaves_[li].s * offd;
-      grow(tmp_, need);
-      std::vector<GemmDesc> g;
-      std::vector<CopyDesc> cp;
-      size_t to = 0;
-      for (int li : level_leaves_[d]) {
-        const LeafH& lf = leaves_[li];
-        g.push_back(gemm_desc(lf.s, offd, lf.s, 0, Linv_.p + lf.mat_off, lf.s, W + lf.row0, ld,
-                              tmp_.p + to, lf.s, 1.0, 0.0));
-        cp.push_back({tmp_.p + to, W + lf.row0, lf.s, offd, lf.s, ld});
-        to += (size_t)lf.s * offd;
-      }
-      run_gemm(g);
-      launch_copy(arena_.push(cp, stream_), (int)cp.size(), stream_);
-    }
+    // leaves of level d: W[rows, 0:off_d] <- A_leaf^-1 W[rows, 0:off_d] (in
+    // place by the LU factors; the columns of level d and deeper are zero
+    // padding in a leaf's rows)
+    if (offd > 0) leaf_solves(level_leaves_[d], W, ld, offd, false);
     if (level_nodes_[d].empty()) continue;
     // nodes of level d
     const int ncols = offd + K_[d];
@@ -1580,20 +1563,11 @@ void Engine::solve_fp64_inplace(int which, double* X, int nrhs) {
   Factorization& F = fac_[which];
   const int ld = ntot_;
   const size_t N = (size_t)ntot_;
-  // leaves: X[rows] <- A_leaf^-1 X[rows]
+  // leaves: X[rows] <- A_leaf^-1 X[rows] (in place by the LU factors)
   {
-    grow(tmp_, N * nrhs);
-    std::vector<GemmDesc> g;
-    std::vector<CopyDesc> cp;
-    size_t to = 0;
-    for (auto& lf : leaves_) {
-      g.push_back(gemm_desc(lf.s, nrhs, lf.s, 0, Linv_.p + lf.mat_off, lf.s, X + lf.row0, ld,
-                            tmp_.p + to, lf.s, 1.0, 0.0));
-      cp.push_back({tmp_.p + to, X + lf.row0, lf.s, nrhs, lf.s, ld});
-      to += (size_t)lf.s * nrhs;
-    }
-    run_gemm(g);
-    launch_copy(arena_.push(cp, stream_), (int)cp.size(), stream_);
+    std::vector<int> all(leaves_.size());
+    for (size_t i = 0; i < all.size(); i++) all[i] = (int)i;
+    leaf_solves(all, X, ld, nrhs, false);
   }
   for (int d = nlev_ - 1; d >= 0; d--) {
     if (level_nodes_[d].empty()) continue;
@@ -1739,8 +1713,8 @@ void Engine::factorize_mixed(int which) {
     launch_balance(arena_.push(bd, stream_), (int)bd.size(), kmax, stream_);
   }
   if (trace_) {
-    std::fprintf(stderr, "[mixed] leaves: non-finite Linv %lld, W %lld\n",
-                 count_nonfinite(Linv_.p, leaf_elems_), count_nonfinite(M.W.p, N * ktot_));
+    std::fprintf(stderr, "[mixed] leaves: non-finite LU %lld, W %lld\n",
+                 count_nonfinite(Lfac_.p, leaf_elems_), count_nonfinite(M.W.p, N * ktot_));
     for (int d = 0; d < nlev_; d++)
       if (K_[d] > 0)
         std::fprintf(stderr, "[mixed] level %d panels: non-finite U %lld, V %lld, V32 %lld\n", d,
@@ -1764,28 +1738,24 @@ void Engine::factorize_mixed(int which) {
   for (int d = nlev_ - 1; d >= 0; d--) {
     const int offd = off_[d];
     if (offd > 0 && !level_leaves_[d].empty()) {
-      // leaf inverses applied in FP64: their condition numbers reach 1e6 on
-      // smooth kernels with small shifts, and applying an FP32-rounded inverse
-      // loses every digit of the (much smaller) result
+      // leaf solves in FP64: the leaves' condition numbers reach 1e6 on
+      // smooth kernels with small shifts, and FP32 solves lose every digit
+      // of the (much smaller) result
       size_t need = 0;
       for (int li : level_leaves_[d]) need += (size_t)leaves_[li].s * offd;
-      grow(tmp2_, 2 * need);
-      std::vector<GemmDesc> g;
+      grow(tmp2_, need);
       std::vector<ConvDesc<float, double>> up;
       std::vector<ConvDesc<double, float>> down;
       size_t to = 0;
       for (int li : level_leaves_[d]) {
         const LeafH& lf = leaves_[li];
         double* x64 = tmp2_.p + to;
-        double* y64 = tmp2_.p + need + to;
         up.push_back({W + lf.row0, x64, lf.s, offd, ld, lf.s});
-        g.push_back(gemm_desc(lf.s, offd, lf.s, 0, Linv_.p + lf.mat_off, lf.s, x64, lf.s, y64, lf.s,
-                              1.0, 0.0));
-        down.push_back({y64, W + lf.row0, lf.s, offd, lf.s, ld});
+        down.push_back({x64, W + lf.row0, lf.s, offd, lf.s, ld});
         to += (size_t)lf.s * offd;
       }
       launch_convert(arena_.push(up, stream_), (int)up.size(), stream_);
-      run_gemm(g);
+      leaf_solves(level_leaves_[d], tmp2_.p, 0, offd, true);
       launch_convert(arena_.push(down, stream_), (int)down.size(), stream_);
       if (trace_)
         std::fprintf(stderr, "[mixed] level %d leaf apply: non-finite W %lld\n", d,
@@ -1885,24 +1855,23 @@ void Engine::solve_mixed_once(int which, float* X, int nrhs) {
   const int ld = ntot_;
   const size_t N = (size_t)ntot_;
   {
-    // leaf inverses in FP64 (see factorize_mixed)
+    // leaf solves in FP64 (see factorize_mixed)
     const size_t need = N * nrhs;
-    grow(tmp2_, 2 * need);
-    std::vector<GemmDesc> g;
+    grow(tmp2_, need);
     std::vector<ConvDesc<float, double>> up;
     std::vector<ConvDesc<double, float>> down;
+    std::vector<int> all(leaves_.size());
     size_t to = 0;
-    for (auto& lf : leaves_) {
+    for (size_t i = 0; i < leaves_.size(); i++) {
+      const LeafH& lf = leaves_[i];
+      all[i] = (int)i;
       double* x64 = tmp2_.p + to;
-      double* y64 = tmp2_.p + need + to;
       up.push_back({X + lf.row0, x64, lf.s, nrhs, ld, lf.s});
-      g.push_back(gemm_desc(lf.s, nrhs, lf.s, 0, Linv_.p + lf.mat_off, lf.s, x64, lf.s, y64, lf.s,
-                            1.0, 0.0));
-      down.push_back({y64, X + lf.row0, lf.s, nrhs, lf.s, ld});
+      down.push_back({x64, X + lf.row0, lf.s, nrhs, lf.s, ld});
       to += (size_t)lf.s * nrhs;
     }
     launch_convert(arena_.push(up, stream_), (int)up.size(), stream_);
-    run_gemm(g);
+    leaf_solves(all, tmp2_.p, 0, nrhs, true);
     launch_convert(arena_.push(down, stream_), (int)down.size(), stream_);
   }
   for (int d = nlev_ - 1; d >= 0; d--) {

@@ -46,6 +46,8 @@ struct Tuning {
   int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
   int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
   int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
+  int leaf_blk = 0;        // leaf solves of >= 128 columns (the factorization's): 1 blocked
+                           // (diagonal-block substitution + GEMM updates), 0 one launch
   bool tuned = false;
 };
 
@@ -290,6 +292,7 @@ class Engine {
   std::vector<int> predicted_capacities();
   void assemble_leaves();
   void factor_leaves();
+  void leaf_solves(const std::vector<int>& leaf_ids, double* B, int ldb, int ncols, bool compact);
   void run_gemm(std::vector<GemmDesc>& descs);
   template <class T>
   void run_gemm_t(std::vector<GemmDescT<T>>& descs);
@@ -313,7 +316,8 @@ class Engine {
   void grow_t(DBuf<T>& b, size_t n);
   template <class T>
   long long count_nonfinite(const T* x, size_t n);
-  void lu_solve_batch(std::vector<SolveDesc>& descs);
+  // blocked: the permutation / block substitution / GEMM path whatever the size
+  void lu_solve_batch(std::vector<SolveDesc>& descs, bool blocked = false);
 
   std::vector<InstanceSpec> specs_;
   int ntot_ = 0, nlev_ = 0;
@@ -370,7 +374,7 @@ class Engine {
   std::vector<DBuf<double>> U_, V_;
   std::vector<int> K_, off_;
   int ktot_ = 0;
-  DBuf<double> Lorig_, Lfac_, Linv_;
+  DBuf<double> Lorig_, Lfac_;
   DBuf<int> Lpiv_;
   DBuf<int> lu_swp_;  // the batched LU's recorded panel interchanges (LU_SWP_INTS per matrix)
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;

@@ -130,6 +130,7 @@ typedef struct hodlr_stats {
   int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
   double factor_stream_time; /* the factorization up to its last kernel on the engine's
                                 stream, where a following solve starts */
+  int tune_leaf_blocked;  /* the factorization's leaf solves blocked (1) or one launch (0) */
 } hodlr_stats;
 
 /* One node of a user-defined partition: user_IndexTree::node
@@ -243,20 +244,24 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
 
 /* Autotuner: times candidate launch shapes on this handle's own problem
  * (ACA panel rows per thread 1/2/4 on the compression; grouped GEMM tile
- * rows, split-K target, vector or matrix-core (MFMA) FP64 kernel and LU panel
- * width on the FP64 factorization), keeps
+ * rows, split-K target, vector or matrix-core (MFMA) FP64 kernel, LU panel
+ * width and blocked or one-launch leaf solves on the FP64 factorization), keeps
  * the fastest, and applies them to this handle and to every handle of the
  * same shape (instance sizes, leaf sizes) created later in the process.
  * Drops cached compression and factorization.  No reference counterpart. */
 int hodlr_autotune(hodlr_handle h);
 
 /* Fixes the launch shapes instead of the autotuner's or the defaults (2, 0,
- * 2048, 64, 0): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
+ * 2048, 64, 0, 0): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
  * (0: per launch, 64, 128), split-K target tiles, LU panel width (1..64),
- * FP64 GEMMs with 128-row tiles on the matrix cores (0, 1).  Drops cached
- * compression and factorization.  No reference counterpart. */
+ * FP64 GEMMs with 128-row tiles on the matrix cores (0, 1), and the
+ * factorization's leaf solves (of hundreds of columns) blocked -- diagonal
+ * blocks by substitution, the rest by GEMM updates -- (1) or in one launch
+ * (0).  Drops cached compression and factorization.  No reference
+ * counterpart. */
 int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_tile_m,
-                            int gemm_split_target, int lu_panel, int gemm_mfma);
+                            int gemm_split_target, int lu_panel, int gemm_mfma,
+                            int leaf_solve_blocked);
 
 /* Replay FP64 solves from captured hipGraphs (off by default).  The level
  * sequence of a recSM/recLU solve with a given number of right-hand sides

@@ -508,6 +508,7 @@ struct Node {
   Mat tU_SM, bU_SM;
   Mat leafMat;
   LU leafLU;
+  Mat leafInv;  // explicit inverse (the leaf-inverse variant only)
   PerturbI pI;
   Mat topSolnLR, bottSolnLR;  // recLU: A11^-1 U_t, A22^-1 U_b
   LU schur;
@@ -524,6 +525,23 @@ struct Hodlr {
   std::vector<std::vector<Node*>> levels;  // non-leaf nodes per level, left to right
   std::vector<Node*> leaves;
   bool compressed = false, smFactored = false, luFactored = false;
+  /* Variant for measurement, not the reference: leaf solves multiply by an
+   * explicitly formed inverse (LU solves of the identity), as the GPU
+   * engine's leaf steps do. */
+  bool leafInverse = false;
+  void leafSolve(Node* c, Mat& rhs) {
+    if (!leafInverse) {
+      c->leafLU.solve_inplace(rhs);
+      return;
+    }
+    if (c->leafInv.r == 0) {
+      const int s = c->leafLU.lu.r;
+      c->leafInv = Mat(s, s);
+      for (int i = 0; i < s; i++) c->leafInv(i, i) = 1;
+      c->leafLU.solve_inplace(c->leafInv);
+    }
+    rhs = matmul(c->leafInv, rhs);
+  }
   double tCompress = 0, tSmFactor = 0, tSmSolve = 0, tLuFactor = 0, tLuSolve = 0;
 
   Node* newNode() {
@@ -711,7 +729,7 @@ struct Hodlr {
       int cs = c->max_i - c->min_i + 1, off = c->min_i - r->min_i;
       Mat rhs = rows_of(r->tU_SM, off, cs);
       if (c->leaf) {
-        c->leafLU.solve_inplace(rhs);
+        leafSolve(c, rhs);
       } else {
         if (!c->pI.built) c->pI.build(c->tU_SM, c->tV, c->bU_SM, c->bV);
         rhs = c->pI.solve(rhs);
@@ -722,7 +740,7 @@ struct Hodlr {
       int cs = c->max_i - c->min_i + 1, off = c->min_i - r->split - 1;
       Mat rhs = rows_of(r->bU_SM, off, cs);
       if (c->leaf) {
-        c->leafLU.solve_inplace(rhs);
+        leafSolve(c, rhs);
       } else {
         if (!c->pI.built) c->pI.build(c->tU_SM, c->tV, c->bU_SM, c->bV);
         rhs = c->pI.solve(rhs);
@@ -748,7 +766,7 @@ struct Hodlr {
     int s = r->max_i - r->min_i + 1;
     if (r->leaf) {
       Mat b = rows_of(X, r->min_i, s);
-      r->leafLU.solve_inplace(b);
+      leafSolve(r, b);
       set_rows(X, r->min_i, b);
       return;
     }
@@ -1173,6 +1191,19 @@ void orc_matvec(void* p, const double* X, int nrhs, double* Y) {
   std::copy(X, X + (size_t)h->n * nrhs, XM.a.begin());
   h->matvec(h->root, XM, YM);
   std::copy(YM.a.begin(), YM.a.end(), Y);
+}
+
+/* Measurement variant (not the reference): leaf solves by explicit inverses. */
+void orc_set_leaf_inverse(void* p, int on) {
+  Hodlr* h = (Hodlr*)p;
+  h->leafInverse = on != 0;
+  h->smFactored = false;
+  for (auto& nd : h->pool) {
+    nd->tU_SM = Mat();
+    nd->bU_SM = Mat();
+    nd->pI = PerturbI();
+    nd->leafInv = Mat();
+  }
 }
 
 /* R = B - A_H X in double-double arithmetic, rounded once. */

@@ -66,6 +66,7 @@ def lib():
         L.orc_logabsdet.argtypes = [_p, _dp]
         L.orc_matvec.argtypes = [_p, _dp, _i, _dp]
         L.orc_residual_dd.argtypes = [_p, _dp, _dp, _i, _dp]
+        L.orc_set_leaf_inverse.argtypes = [_p, _i]
         L.orc_refined_solve.restype = _i
         L.orc_refined_solve.argtypes = [_p, _dp, _i, _dp, _d, _i, _dp]
         L.orc_time.restype = _d
@@ -243,6 +244,11 @@ class OracleHODLR:
         Y = np.zeros_like(X, order="F")
         lib().orc_matvec(self.h, _ptr(X), X.shape[1], _ptr(Y))
         return Y
+
+    def set_leaf_inverse(self, on=True):
+        """Measurement variant, not the reference: recSM's leaf solves multiply
+        by explicitly formed inverses (as the GPU engine's leaf steps do)."""
+        lib().orc_set_leaf_inverse(self.h, int(bool(on)))
 
     def residual_dd(self, B, X):
         """B - A_H X in double-double arithmetic, rounded once."""
