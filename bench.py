@@ -55,8 +55,8 @@ def parse():
     p.add_argument("--no-autotune", action="store_true")
     p.add_argument("--launch-shapes", default=None,
                    help="fix the launch shapes instead of autotuning: ACA rows per thread, "
-                        "GEMM tile rows, split-K target, LU panel, MFMA, blocked leaf solves "
-                        "(e.g. 4,0,4096,32,1,1)")
+                        "GEMM tile rows, split-K target, LU panel, MFMA, leaf-solve variant "
+                        "(e.g. 4,0,4096,32,1,0)")
     p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
     p.add_argument("--workload", default="gauss2d_262144", choices=["gauss2d_262144", "gp_batch_512"],
                    help="gp_batch_512: the batched Gaussian-process systems instead of the "
@@ -116,8 +116,7 @@ def make_handle(pts, target, precision, max_steps):
     from dense_hodlr_amd import HODLR_Matrix
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
     if SHAPES:
-        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]),
-                            bool(SHAPES[5]) if len(SHAPES) > 5 else False)
+        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 0)
     g.set_LRTolerance(1e-10)
     g.set_precision(precision)
     g.set_refinement(target, max_steps)

@@ -90,7 +90,7 @@ class Stats(C.Structure):
         ("tuned", C.c_int),
         ("tune_gemm_mfma", C.c_int),
         ("factor_stream_time", C.c_double),
-        ("tune_leaf_blocked", C.c_int),
+        ("tune_leaf_solve", C.c_int),
     ]
 
 

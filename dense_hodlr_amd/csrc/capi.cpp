@@ -431,7 +431,7 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters) {
 
 int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_tile_m,
                             int gemm_split_target, int lu_panel, int gemm_mfma,
-                            int leaf_solve_blocked) {
+                            int leaf_solve) {
   return guard(h, [&] {
     hodlr::Tuning t;
     t.aca_pt = aca_rows_per_thread;
@@ -439,7 +439,7 @@ int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_ti
     t.gemm_target = gemm_split_target;
     t.lu_nb = lu_panel;
     t.gemm_mfma = gemm_mfma;
-    t.leaf_blk = leaf_solve_blocked;
+    t.leaf_solve = leaf_solve;
     h->eng->set_tuning(t);
   });
 }
@@ -643,7 +643,7 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->tune_gemm_split_target = tu.gemm_target;
     out->tune_lu_panel = tu.lu_nb;
     out->tune_gemm_mfma = tu.gemm_mfma;
-    out->tune_leaf_blocked = tu.leaf_blk;
+    out->tune_leaf_solve = tu.leaf_solve;
     out->tuned = tu.tuned ? 1 : 0;
     out->graph_captures = s.graph_captures;
     out->graph_launches = s.graph_launches;

@@ -168,6 +168,13 @@ void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 constexpr int LU_SOLVE_SMALL_MAX = 2048;
 constexpr int LU_SOLVE_WAVE_MAX = 512;
 int lu_solve_small_wgs(int ncols, int maxn, bool wave);
+/* The same solves for many right-hand sides of systems of at most
+ * LU_SOLVE_WAVE_MAX rows, the factor shared by the four waves of a workgroup
+ * (bit-identical to the one-wave kernel); SolveDesc::chunk0 numbers
+ * workgroups, lu_solve_shared_wgs(ncols, maxn) per problem.  Returns false,
+ * and launches nothing, when maxn exceeds the limit. */
+int lu_solve_shared_wgs(int ncols, int maxn);
+bool launch_lu_solve_shared(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
                            bool blocked, hipStream_t s);
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,

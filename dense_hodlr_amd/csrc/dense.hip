@@ -1040,6 +1040,141 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
   }
 }
 
+/* Solves of many right-hand sides (the factorization's leaf steps, hundreds
+ * of columns) for systems of at most 512 rows.  One workgroup takes four
+ * consecutive column chunks of one system, one per wave, and the waves share
+ * the factor: its columns are staged through LDS, LU_SHARED_PIECE at a time,
+ * and read from there, so each factor column is loaded once per four chunks
+ * instead of once per chunk (per chunk the one-wave kernel streams the whole
+ * factor).  Per element the operations and their order are those of
+ * lu_solve_wave_kernel (the interchanges as one permutation, the forward
+ * steps in increasing order, the backward steps with one division by the
+ * diagonal each), so results are bit-identical to it. */
+constexpr int LU_SHARED_PIECE = 16;
+
+template <int R, int CB>
+__global__ void __launch_bounds__(TPB) lu_solve_shared_kernel(const SolveDesc* __restrict__ d,
+                                                              int count) {
+  constexpr int PS = LU_SHARED_PIECE;
+  constexpr int LDR = 64 * R;
+  __shared__ double Ls[PS][LDR];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= (int)blockIdx.x)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SolveDesc p = d[lo];
+  const int n = p.n;
+  const int c0 = ((int)blockIdx.x - p.chunk0) * (4 * CB) + w * CB;
+  const int nc = min(CB, p.ncols - c0);  // (<= 0: the wave only takes part in the barriers)
+  int pm[R], pvr[R];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+    pm[r] = i;
+    pvr[r] = i < n ? p.piv[i] : i;
+  }
+#pragma unroll
+  for (int g = 0; g < R; g++)
+    for (int k = 64 * g; k < min(n, 64 * g + 64); k++) {
+      const int q = __builtin_amdgcn_readlane(pvr[g], k - 64 * g);
+      if (q != k && (unsigned)q < (unsigned)n) {
+        const int a = __builtin_amdgcn_readlane(pm[g], k - 64 * g);
+        int b = a;
+#pragma unroll
+        for (int r = 0; r < R; r++)
+          if (r == (q >> 6)) b = __builtin_amdgcn_readlane(pm[r], q & 63);
+#pragma unroll
+        for (int r = 0; r < R; r++) {
+          const int i = lane + 64 * r;
+          if (i == k) pm[r] = b;
+          if (i == q) pm[r] = a;
+        }
+      }
+    }
+  double x[R][CB];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+#pragma unroll
+    for (int c = 0; c < CB; c++)
+      x[r][c] = (i < n && c < nc) ? p.B[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
+  }
+  const double* LU = p.LU;
+  const size_t ld = (size_t)p.ld_lu;
+  // forward: unit lower triangle, the columns l of row group g in pieces
+#pragma unroll
+  for (int g = 0; g < R; g++)
+    for (int l0 = 64 * g; l0 < min(n, 64 * g + 64); l0 += PS) {
+      __syncthreads();
+      for (int e = t; e < PS * LDR; e += TPB) {
+        const int j = e / LDR, i = e - j * LDR, l = l0 + j;
+        Ls[j][i] = (l < n && i > l && i < n) ? LU[(size_t)i + (size_t)l * ld] : 0.0;
+      }
+      __syncthreads();
+      for (int s = 0; s < PS; s++) {
+        const int l = l0 + s;
+        if (l >= n) break;
+        double xl[CB];
+#pragma unroll
+        for (int c = 0; c < CB; c++) xl[c] = wave_bcast(x[g][c], l - 64 * g);
+#pragma unroll
+        for (int r = g; r < R; r++) {
+          const int i = lane + 64 * r;
+          if (i > l && i < n) {
+            const double f = Ls[s][i];
+#pragma unroll
+            for (int c = 0; c < CB; c++) x[r][c] -= f * xl[c];
+          }
+        }
+      }
+    }
+  // backward: upper triangle with its diagonal, from the last column down
+#pragma unroll
+  for (int g = R - 1; g >= 0; g--)
+    for (int lh = min(n - 1, 64 * g + 63); lh >= 64 * g; lh -= PS) {
+      __syncthreads();
+      for (int e = t; e < PS * LDR; e += TPB) {
+        const int j = e / LDR, i = e - j * LDR, l = lh - j;
+        Ls[j][i] = (l >= 64 * g && i <= l) ? LU[(size_t)i + (size_t)l * ld] : 0.0;
+      }
+      __syncthreads();
+      for (int s = 0; s < PS; s++) {
+        const int l = lh - s;
+        if (l < 64 * g) break;
+        const int own = l - 64 * g;
+        const double dg = Ls[s][l];
+        double xl[CB];
+#pragma unroll
+        for (int c = 0; c < CB; c++) xl[c] = wave_bcast(x[g][c], own) / dg;
+        if (lane == own)
+#pragma unroll
+          for (int c = 0; c < CB; c++) x[g][c] = xl[c];
+#pragma unroll
+        for (int r = 0; r <= g; r++) {
+          const int i = lane + 64 * r;
+          if (i < l) {
+            const double f = Ls[s][i];
+#pragma unroll
+            for (int c = 0; c < CB; c++) x[r][c] -= f * xl[c];
+          }
+        }
+      }
+    }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const int i = lane + 64 * r;
+    if (i < n)
+#pragma unroll
+      for (int c = 0; c < CB; c++)
+        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+  }
+}
+
 /* The same solves for systems of 512-2048 rows, in blocks of 64 rows: the
  * wave that holds a block's rows solves the block's diagonal part alone
  * (staged in LDS, the step's value broadcast by readlane, no barrier), then
@@ -1259,6 +1394,24 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
   trsm_kernel<<<nwg, 64, 0, s>>>(d, count);
   HODLR_HIP_CHECK(hipGetLastError());
+}
+
+int lu_solve_shared_wgs(int ncols, int maxn) {
+  const int cb = 4 * wave_cols(wave_rows(maxn));
+  return (ncols + cb - 1) / cb;
+}
+
+bool launch_lu_solve_shared(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s) {
+  if (maxn > LU_SOLVE_WAVE_MAX) return false;
+  if (count == 0 || nwg == 0) return true;
+  switch (wave_rows(maxn)) {
+    case 1: lu_solve_shared_kernel<1, 8><<<nwg, TPB, 0, s>>>(d, count); break;
+    case 2: lu_solve_shared_kernel<2, 8><<<nwg, TPB, 0, s>>>(d, count); break;
+    case 4: lu_solve_shared_kernel<4, 8><<<nwg, TPB, 0, s>>>(d, count); break;
+    default: lu_solve_shared_kernel<8, 4><<<nwg, TPB, 0, s>>>(d, count); break;
+  }
+  HODLR_HIP_CHECK(hipGetLastError());
+  return true;
 }
 
 int lu_solve_small_wgs(int ncols, int maxn, bool wave) {

@@ -114,8 +114,8 @@ void Engine::set_tuning(const Tuning& t) {
   if (t.gemm_target < 1) throw std::invalid_argument("split-K target must be positive");
   if (t.lu_nb < 1 || t.lu_nb > LU_NB) throw std::invalid_argument("LU panel width must be 1..64");
   if (t.gemm_mfma != 0 && t.gemm_mfma != 1) throw std::invalid_argument("mfma must be 0 or 1");
-  if (t.leaf_blk != 0 && t.leaf_blk != 1)
-    throw std::invalid_argument("blocked leaf solves must be 0 or 1");
+  if (t.leaf_solve < 0 || t.leaf_solve > 2)
+    throw std::invalid_argument("the leaf solve variant must be 0, 1 or 2");
   tune_ = t;
   tune_.tuned = false;
   reset_attributes();
@@ -167,13 +167,17 @@ void Engine::autotune() {
         fb = t;
       }
     }
-  {  // the leaf solves of the factorization: one launch or blocked
-    Tuning t = fb;
-    t.leaf_blk = 1 - fb.leaf_blk;
-    const double v = std::min(time_factor(t), time_factor(t));
-    if (v < bt) {
-      bt = v;
-      fb = t;
+  {  // the leaf solves of the factorization: shared factor, blocked, one wave
+    const Tuning base = fb;
+    for (int ls : {0, 1, 2}) {
+      if (ls == base.leaf_solve) continue;
+      Tuning t = base;
+      t.leaf_solve = ls;
+      const double v = std::min(time_factor(t), time_factor(t));
+      if (v < bt) {
+        bt = v;
+        fb = t;
+      }
     }
   }
   fb.tuned = true;
@@ -643,11 +647,20 @@ void Engine::lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& aren
 /* B <- A^-1 B for a batch of LU-factored matrices: row interchanges, then
  * blocked forward (unit L) and backward (U) substitution, each block a small
  * triangular solve followed by a grouped GEMM update of the remaining rows. */
-void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, bool blocked) {
+void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, int mode) {
   if (descs.empty()) return;
   int maxn = 0;
   for (auto& d : descs) maxn = std::max(maxn, d.n);
-  if (maxn <= solve_small_max_ && !blocked) {  // every system in one launch
+  if (mode == 2 && maxn <= LU_SOLVE_WAVE_MAX && maxn <= solve_wave_max_) {
+    int nwg = 0;
+    for (auto& d : descs) {
+      d.chunk0 = nwg;
+      nwg += lu_solve_shared_wgs(d.ncols, maxn);
+    }
+    launch_lu_solve_shared(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, stream_);
+    return;
+  }
+  if (maxn <= solve_small_max_ && mode != 1) {  // every system in one launch
     const bool wave = maxn <= solve_wave_max_;
     int nwg = 0;
     for (auto& d : descs) {
@@ -1317,8 +1330,9 @@ void Engine::leaf_solves(const std::vector<int>& leaf_ids, double* B, int ldb, i
     sd.push_back(s);
     o += (size_t)lf.s * ncols;
   }
-  // many columns (the factorization's leaf step): blocked if the tuning says so
-  lu_solve_batch(sd, tune_.leaf_blk != 0 && ncols >= 128);
+  // many columns (the factorization's leaf step): the tuned variant
+  const int mode = ncols < 128 ? 0 : tune_.leaf_solve;
+  lu_solve_batch(sd, mode);
 }
 
 /* recSM_Factorize (HODLR_Matrix.cpp:520-592) and recLU_Factorize (:612-692),

@@ -46,8 +46,9 @@ struct Tuning {
   int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
   int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
   int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
-  int leaf_blk = 0;        // leaf solves of >= 128 columns (the factorization's): 1 blocked
-                           // (diagonal-block substitution + GEMM updates), 0 one launch
+  int leaf_solve = 0;      // leaf solves of >= 128 columns (the factorization's): 0 one
+                           // wave per column chunk, 1 blocked (diagonal-block substitution +
+                           // GEMM updates), 2 the factor shared by a workgroup's waves
   bool tuned = false;
 };
 
@@ -316,8 +317,10 @@ class Engine {
   void grow_t(DBuf<T>& b, size_t n);
   template <class T>
   long long count_nonfinite(const T* x, size_t n);
-  // blocked: the permutation / block substitution / GEMM path whatever the size
-  void lu_solve_batch(std::vector<SolveDesc>& descs, bool blocked = false);
+  // mode 0: by size (one launch up to 2048 rows, else blocked); 1: the
+  // permutation / block substitution / GEMM path whatever the size; 2: the
+  // factor shared by a workgroup's waves (systems of at most 512 rows)
+  void lu_solve_batch(std::vector<SolveDesc>& descs, int mode = 0);
 
   std::vector<InstanceSpec> specs_;
   int ntot_ = 0, nlev_ = 0;

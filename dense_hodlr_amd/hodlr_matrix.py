@@ -152,11 +152,11 @@ class HODLR_Matrix:
         check(lib().hodlr_autotune(self._h), self._h)
 
     def set_launch_shapes(self, aca_rows_per_thread=2, gemm_tile_m=0, gemm_split_target=2048,
-                          lu_panel=64, gemm_mfma=False, leaf_solve_blocked=False):
+                          lu_panel=64, gemm_mfma=False, leaf_solve=0):
         """Fix the launch shapes (see hodlr_set_launch_shapes)."""
         check(lib().hodlr_set_launch_shapes(self._h, int(aca_rows_per_thread), int(gemm_tile_m),
                                             int(gemm_split_target), int(lu_panel),
-                                            int(bool(gemm_mfma)), int(bool(leaf_solve_blocked))),
+                                            int(bool(gemm_mfma)), int(leaf_solve)),
               self._h)
 
     def set_graphs(self, on: bool = True):
@@ -297,7 +297,7 @@ class HODLR_Matrix:
                                 gemm_tile_m=s.tune_gemm_tile_m,
                                 gemm_split_target=s.tune_gemm_split_target,
                                 lu_panel=s.tune_lu_panel, gemm_mfma=bool(s.tune_gemm_mfma),
-                                leaf_solve_blocked=bool(s.tune_leaf_blocked),
+                                leaf_solve=s.tune_leaf_solve,
                                 tuned=bool(s.tuned)))
 
     def get_LR_ComputationTime(self) -> float:

@@ -130,7 +130,8 @@ typedef struct hodlr_stats {
   int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
   double factor_stream_time; /* the factorization up to its last kernel on the engine's
                                 stream, where a following solve starts */
-  int tune_leaf_blocked;  /* the factorization's leaf solves blocked (1) or one launch (0) */
+  int tune_leaf_solve;    /* the factorization's leaf solves: 0 one wave per column chunk,
+                             1 blocked, 2 factor shared per workgroup */
 } hodlr_stats;
 
 /* One node of a user-defined partition: user_IndexTree::node
@@ -245,7 +246,7 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
 /* Autotuner: times candidate launch shapes on this handle's own problem
  * (ACA panel rows per thread 1/2/4 on the compression; grouped GEMM tile
  * rows, split-K target, vector or matrix-core (MFMA) FP64 kernel, LU panel
- * width and blocked or one-launch leaf solves on the FP64 factorization), keeps
+ * width and the leaf-solve variant on the FP64 factorization), keeps
  * the fastest, and applies them to this handle and to every handle of the
  * same shape (instance sizes, leaf sizes) created later in the process.
  * Drops cached compression and factorization.  No reference counterpart. */
@@ -255,13 +256,13 @@ int hodlr_autotune(hodlr_handle h);
  * 2048, 64, 0, 0): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
  * (0: per launch, 64, 128), split-K target tiles, LU panel width (1..64),
  * FP64 GEMMs with 128-row tiles on the matrix cores (0, 1), and the
- * factorization's leaf solves (of hundreds of columns) blocked -- diagonal
- * blocks by substitution, the rest by GEMM updates -- (1) or in one launch
- * (0).  Drops cached compression and factorization.  No reference
- * counterpart. */
+ * variant of the factorization's leaf solves (of 128 or more columns): 0 one
+ * launch, one wave per column chunk, 1 blocked (diagonal blocks by
+ * substitution, the rest by GEMM updates), 2 one launch with the factor
+ * shared by the waves of a workgroup.  Drops cached compression and factorization.
+ * No reference counterpart. */
 int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_tile_m,
-                            int gemm_split_target, int lu_panel, int gemm_mfma,
-                            int leaf_solve_blocked);
+                            int gemm_split_target, int lu_panel, int gemm_mfma, int leaf_solve);
 
 /* Replay FP64 solves from captured hipGraphs (off by default).  The level
  * sequence of a recSM/recLU solve with a given number of right-hand sides

@@ -390,7 +390,9 @@ def test_svd_method_gauss1d_4096(gauss1d):
     dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=1024, lu_panel=32,
          gemm_mfma=True),
     dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=4096, lu_panel=32,
-         gemm_mfma=True, leaf_solve_blocked=True),
+         gemm_mfma=True, leaf_solve=1),
+    dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=4096, lu_panel=32,
+         gemm_mfma=True, leaf_solve=2),
 ])
 def test_launch_shapes_match_oracle(oracle, shapes):
     """Every launch-shape variant (ACA rows per thread, vector or matrix-core
@@ -502,3 +504,29 @@ def test_matern2d_65536_against_oracle_summary():
     assert dnr < 1e-12 and relr < 1e-10
     ld = g.logAbsDeterminant()
     assert abs(ld - float(f["logdet"])) <= 1e-12 * abs(float(f["logdet"]))
+
+
+@pytest.mark.parametrize("leaf", [40, 100, 200, 400])
+def test_shared_factor_leaf_solves_match_wave_kernel(leaf):
+    """The factorization's leaf steps (hundreds of columns) solve with one
+    wave per 8-column chunk streaming the whole factor (leaf_solve 0, the
+    default) or with the factor shared by a workgroup's four waves (2):
+    the same operations in the same order per element, so recSM, recLU and
+    the log-det are bit-identical.  Leaf sizes 40/100/200/400 (ragged trees
+    of 6000 points) cover one, two, four and eight rows per lane."""
+    _case(f"shared-factor leaf solves vs one wave, leaf {leaf}")
+    n = 6000
+    pts = I.points_2d_kd(n, 43)
+    b = I.rhs_normal(n, 3, 44)
+    out = []
+    for ls in (0, 2):
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.06, 1e-3, sizeThreshold=leaf)
+        g.set_launch_shapes(leaf_solve=ls)
+        g.set_LRTolerance(1e-10)
+        out.append((g.recSM_Solve(b), g.recLU_Solve(b), g.logAbsDeterminant(), g.nodes()))
+        del g
+    (xs0, xl0, d0, n0), (xs1, xl1, d1, n1) = out
+    assert np.array_equal(n0, n1)
+    assert np.sum(n0[:, 4:]) > 128  # the leaf steps have many columns
+    assert np.array_equal(xs0, xs1) and np.array_equal(xl0, xl1)
+    assert d0 == d1
