@@ -45,6 +45,7 @@ EXPORTS = [
     "hodlr_read_binary_matrix", "hodlr_write_binary_matrix", "hodlr_read_text_matrix",
     "hodlr_create_dense_from_file", "hodlr_create_kernel_tree", "hodlr_create_dense_tree",
     "hodlr_kd_sort", "hodlr_kd_tree", "hodlr_diag_block", "hodlr_split_at_top",
+    "hodlr_iterative_solve", "hodlr_iterative_solve_device", "hodlr_exact_apply_device",
 ]
 
 
@@ -91,6 +92,9 @@ class Stats(C.Structure):
         ("tune_gemm_mfma", C.c_int),
         ("factor_stream_time", C.c_double),
         ("tune_leaf_solve", C.c_int),
+        ("iter_iterations", C.c_int),
+        ("iter_precond_residual", C.c_double),
+        ("iter_residual", C.c_double),
     ]
 
 
@@ -197,6 +201,13 @@ def lib():
         "hodlr_create_dense_tree": (C.c_int, [hp, C.c_int, dp, C.c_int, C.c_int, ip, C.c_int]),
         "hodlr_kd_sort": (C.c_int, [dp, C.c_int, C.c_int, ip]),
         "hodlr_kd_tree": (C.c_int, [dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, ip, ip, ip]),
+        "hodlr_iterative_solve": (C.c_int, [h, dp, C.c_int, C.c_int, dp, C.c_int, C.c_int,
+                                            C.c_double, C.c_double, C.c_int, C.c_int]),
+        "hodlr_iterative_solve_device": (C.c_int, [h, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                                   C.c_int, C.c_int, C.c_double, C.c_double,
+                                                   C.c_int, C.c_int]),
+        "hodlr_exact_apply_device": (C.c_int, [h, C.c_void_p, C.c_int, C.c_int, C.c_void_p,
+                                               C.c_int]),
         "hodlr_diag_block": (C.c_int, [h, C.c_int, hp]),
         "hodlr_split_at_top": (C.c_int, [h, hp, hp]),
         "hodlr_memory_get_stats": (C.c_int, [C.POINTER(MemoryStats)]),

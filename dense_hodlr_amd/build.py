@@ -28,7 +28,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-SOURCES = ["aca.hip", "dense.hip", "gemm.hip", "ddres.hip", "svd.hip", "mlp.hip", "noalloc.hip", "matrix_io.cpp", "access.cpp", "kd.cpp", "pool.cpp", "engine.cpp", "model.cpp", "capi.cpp"]
+SOURCES = ["aca.hip", "dense.hip", "gemm.hip", "ddres.hip", "svd.hip", "mlp.hip", "noalloc.hip", "matrix_io.cpp", "access.cpp", "kd.cpp", "exact.hip", "iterative.cpp", "pool.cpp", "engine.cpp", "model.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -46,7 +46,8 @@ def build(verbose: bool = True) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", "-o", tmp,
+           *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")

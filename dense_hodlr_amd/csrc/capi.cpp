@@ -481,6 +481,69 @@ int hodlr_solve_device(hodlr_handle h, int which, const double* dB, int ldb, int
     h->eng->solve_device(which, dB, ldb, nrhs, dX, ldx);
   });
 }
+namespace {
+void iterative_prepare(hodlr_handle h, int max_iterations, double stop_tolerance,
+                       double init_lr_tolerance, int lr_method, int direct_method) {
+  if (max_iterations < 0 || !(stop_tolerance >= 0.0))
+    throw std::invalid_argument("bad iteration settings");
+  if (direct_method != HODLR_FACTOR_SM && direct_method != HODLR_FACTOR_LU)
+    throw std::invalid_argument("unknown direct solver");
+  // HODLR_Matrix.cpp:1141-1145: the handle keeps these settings afterwards
+  h->eng->set_tolerance(init_lr_tolerance);
+  h->eng->set_lr_method(lr_method);
+}
+}  // namespace
+
+int hodlr_iterative_solve(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx,
+                          int max_iterations, double stop_tolerance, double init_lr_tolerance,
+                          int lr_method, int direct_method) {
+  return guard(h, [&] {
+    const int n = h->eng->rows();
+    if (!B || !X || nrhs < 0 || ldb < n || ldx < n) throw std::invalid_argument("bad arguments");
+    iterative_prepare(h, max_iterations, stop_tolerance, init_lr_tolerance, lr_method,
+                      direct_method);
+    if (nrhs == 0) return;
+    hodlr::DBuf<double> db, dx;
+    db.alloc((size_t)n * nrhs);
+    dx.alloc((size_t)n * nrhs);
+    HODLR_HIP_CHECK(hipMemcpy2D(db.p, (size_t)n * 8, B, (size_t)ldb * 8, (size_t)n * 8, nrhs,
+                                hipMemcpyHostToDevice));
+    auto back = [&] {
+      HODLR_HIP_CHECK(hipMemcpy2D(X, (size_t)ldx * 8, dx.p, (size_t)n * 8, (size_t)n * 8, nrhs,
+                                  hipMemcpyDeviceToHost));
+    };
+    try {
+      h->eng->iterative_solve(direct_method, db.p, n, nrhs, dx.p, n, max_iterations,
+                              stop_tolerance);
+    } catch (const hodlr::NotConverged&) {
+      back();
+      throw;
+    }
+    back();
+  });
+}
+
+int hodlr_iterative_solve_device(hodlr_handle h, const double* dB, int ldb, int nrhs, double* dX,
+                                 int ldx, int max_iterations, double stop_tolerance,
+                                 double init_lr_tolerance, int lr_method, int direct_method) {
+  return guard(h, [&] {
+    if (!dB || !dX || nrhs < 0) throw std::invalid_argument("bad arguments");
+    iterative_prepare(h, max_iterations, stop_tolerance, init_lr_tolerance, lr_method,
+                      direct_method);
+    h->eng->iterative_solve(direct_method, dB, ldb, nrhs, dX, ldx, max_iterations, stop_tolerance);
+  });
+}
+
+int hodlr_exact_apply_device(hodlr_handle h, const double* dX, int ldx, int nrhs, double* dY,
+                             int ldy) {
+  return guard(h, [&] {
+    const int n = h->eng->rows();
+    if (!dX || !dY || nrhs < 0 || ldx < n || ldy < n) throw std::invalid_argument("bad arguments");
+    h->eng->exact_apply(dX, ldx, nrhs, dY, ldy);
+    HODLR_HIP_CHECK(hipStreamSynchronize(h->eng->stream()));
+  });
+}
+
 int hodlr_logabsdet(hodlr_handle h, double* out) {
   return guard(h, [&] {
     if (!out) throw std::invalid_argument("null output");
@@ -653,6 +716,9 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->compress_time = s.compress_time;
     out->factor_time = s.factor_time;
     out->factor_stream_time = s.factor_stream_time;
+    out->iter_iterations = s.iter_iterations;
+    out->iter_precond_residual = s.iter_precond_residual;
+    out->iter_residual = s.iter_residual;
     out->solve_time = s.solve_time;
     out->refine_time = s.refine_time;
     out->refine_iterations = s.refine_iterations;

@@ -18,37 +18,6 @@ namespace hodlr {
 /* ------------------------------------------------------------------ */
 /* DescArena                                                           */
 /* ------------------------------------------------------------------ */
-template <class T>
-T* DescArena::push(const std::vector<T>& v, hipStream_t s) {
-  const size_t bytes = v.size() * sizeof(T);
-  const size_t need = (bytes + 255) & ~size_t(255);
-  if (need == 0) return nullptr;
-  if (replay_) {
-    if (next_ >= keep_.size() || keep_[next_].size() != bytes ||
-        std::memcmp(keep_[next_].data(), v.data(), bytes) != 0)
-      throw std::runtime_error("captured solve does not match its recording");
-    return reinterpret_cast<T*>(dev_[next_++]);
-  }
-  if (chunks_.empty() || chunks_.back().used + need > chunks_.back().buf.n) {
-    Chunk c;
-    c.buf.alloc(std::max<size_t>(need, 8u << 20));
-    chunks_.push_back(std::move(c));
-  }
-  Chunk& c = chunks_.back();
-  keep_.emplace_back(reinterpret_cast<const char*>(v.data()),
-                     reinterpret_cast<const char*>(v.data()) + bytes);
-  char* dst = c.buf.p + c.used;
-  dev_.push_back(dst);
-  HODLR_HIP_CHECK(hipMemcpyAsync(dst, keep_.back().data(), bytes, hipMemcpyHostToDevice, s));
-  c.used += need;
-  return reinterpret_cast<T*>(dst);
-}
-
-// (push is used by other files of the engine for these descriptor types)
-template AssembleDesc* DescArena::push<AssembleDesc>(const std::vector<AssembleDesc>&, hipStream_t);
-template CopyDesc* DescArena::push<CopyDesc>(const std::vector<CopyDesc>&, hipStream_t);
-template LrFillDesc* DescArena::push<LrFillDesc>(const std::vector<LrFillDesc>&, hipStream_t);
-
 void DescArena::reset() {
   for (auto& c : chunks_) c.used = 0;
   while (chunks_.size() > 1) chunks_.pop_front();

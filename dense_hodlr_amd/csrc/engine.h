@@ -20,6 +20,7 @@
 #ifndef DENSE_HODLR_ENGINE_H
 #define DENSE_HODLR_ENGINE_H
 
+#include <cstring>
 #include <deque>
 #include <memory>
 #include <string>
@@ -35,6 +36,9 @@
 namespace hodlr {
 
 constexpr int LU_NB = 64;  // largest panel width of the blocked LU and triangular solves
+
+/* GMRES restart length of the iterative solver (the reference's, HODLR_Matrix.cpp:1181). */
+constexpr int ITER_RESTART = 32;
 
 /* Size of the per-GPU device array behind Engine::matvec_noalloc. */
 constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(64) << 20;
@@ -116,6 +120,32 @@ class DescArena {
   bool replay_ = false;
   size_t next_ = 0;
 };
+
+template <class T>
+inline T* DescArena::push(const std::vector<T>& v, hipStream_t s) {
+  const size_t bytes = v.size() * sizeof(T);
+  const size_t need = (bytes + 255) & ~size_t(255);
+  if (need == 0) return nullptr;
+  if (replay_) {
+    if (next_ >= keep_.size() || keep_[next_].size() != bytes ||
+        std::memcmp(keep_[next_].data(), v.data(), bytes) != 0)
+      throw std::runtime_error("captured solve does not match its recording");
+    return reinterpret_cast<T*>(dev_[next_++]);
+  }
+  if (chunks_.empty() || chunks_.back().used + need > chunks_.back().buf.n) {
+    Chunk c;
+    c.buf.alloc(std::max<size_t>(need, 8u << 20));
+    chunks_.push_back(std::move(c));
+  }
+  Chunk& c = chunks_.back();
+  keep_.emplace_back(reinterpret_cast<const char*>(v.data()),
+                     reinterpret_cast<const char*>(v.data()) + bytes);
+  char* dst = c.buf.p + c.used;
+  dev_.push_back(dst);
+  HODLR_HIP_CHECK(hipMemcpyAsync(dst, keep_.back().data(), bytes, hipMemcpyHostToDevice, s));
+  c.used += need;
+  return reinterpret_cast<T*>(dst);
+}
 
 /* A captured hipGraph of the FP64 recSM/recLU solve for one (factorization,
  * number of right-hand sides), working in place on the engine's buffer gx_. */
@@ -224,6 +254,15 @@ class Engine {
   static void noalloc_setup();
   void residual_dd(const double* dB, int ldb, const double* dX, int ldx, int nrhs, double* dR,
                    int ldr);
+  /* Y = A X with the exact operator: the dense source, or the kernel
+   * evaluated on the fly (per instance). */
+  void exact_apply(const double* dX, int ldx, int nrhs, double* dY, int ldy);
+  /* GMRES(ITER_RESTART) on the exact operator, left-preconditioned by the
+   * FP64 recSM (which 0) or recLU (1) solve, from x0 = M^-1 b, until every
+   * column's preconditioned relative residual is below tol or max_iters
+   * iterations are spent (then NotConverged, the iterate in dX). */
+  void iterative_solve(int which, const double* dB, int ldb, int nrhs, double* dX, int ldx,
+                       int max_iters, double tol);
 
   // inspection
   int rows() const { return ntot_; }
@@ -256,6 +295,9 @@ class Engine {
     int aca_launches = 0;
     double aca_bytes = 0;
     int graph_captures = 0, graph_launches = 0;
+    int iter_iterations = 0;         // iterative solve: GMRES iterations,
+    double iter_precond_residual = 0;  // final ||M^-1 (b - A x)|| / ||M^-1 b||,
+    double iter_residual = 0;          // and ||b - A x|| / ||b|| (largest column)
     int pred_blocks = 0, pred_misses = 0;  // with a rank predictor attached
     double pred_log_err = 0;               // mean |log predicted - log ACA rank|
   };

@@ -192,6 +192,30 @@ class HODLR_Matrix:
     def recLU_Solve(self, input_RHS):
         return self._solve(lib().hodlr_solve_lu, input_RHS)
 
+    def iterative_Solve(self, input_RHS, maxIterations: int = 1000, stop_tolerance: float = 1e-10,
+                        init_LRTolerance: float = 1e-2, input_LRMethod: str = "partialPiv_ACA",
+                        directSolve_Method: str = "recLU"):
+        """iterative_Solve (HODLR_Matrix.hpp:219): GMRES(32) on the exact
+        operator, left-preconditioned by the HODLR solve compressed at
+        init_LRTolerance.  Unlike the reference, maxIterations and
+        stop_tolerance are honoured (its fixed values are the defaults)."""
+        if input_LRMethod not in _lib.LR_METHODS:
+            raise ValueError(f"Invalid low-rank approximation scheme ({input_LRMethod})")
+        which = {"recLU": _lib.FACTOR_LU, "recSM": _lib.FACTOR_SM}.get(directSolve_Method)
+        if which is None:
+            raise ValueError(f"Unknown direct solver type ({directSolve_Method})")
+        B = _as_f64_2d(input_RHS, self._n)
+        X = np.zeros_like(B, order="F")
+        try:
+            check(lib().hodlr_iterative_solve(self._h, _dptr(B), self._n, B.shape[1], _dptr(X),
+                                              self._n, int(maxIterations), float(stop_tolerance),
+                                              float(init_LRTolerance),
+                                              _lib.LR_METHODS[input_LRMethod], which), self._h)
+        except HodlrNotConverged as e:
+            e.x = X
+            raise
+        return X
+
     def logAbsDeterminant(self) -> float:
         out = C.c_double()
         check(lib().hodlr_logabsdet(self._h, C.byref(out)), self._h)
@@ -292,7 +316,9 @@ class HODLR_Matrix:
                     aca_launches=s.aca_launches, aca_bytes=s.aca_bytes,
                     pred_blocks=s.pred_blocks, pred_misses=s.pred_misses,
                     pred_log_err=s.pred_log_err, graph_captures=s.graph_captures,
-                    graph_launches=s.graph_launches,
+                    graph_launches=s.graph_launches, iter_iterations=s.iter_iterations,
+                    iter_precond_residual=s.iter_precond_residual,
+                    iter_residual=s.iter_residual,
                     tuning=dict(aca_rows_per_thread=s.tune_aca_rows_per_thread,
                                 gemm_tile_m=s.tune_gemm_tile_m,
                                 gemm_split_target=s.tune_gemm_split_target,

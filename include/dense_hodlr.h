@@ -132,6 +132,12 @@ typedef struct hodlr_stats {
                                 stream, where a following solve starts */
   int tune_leaf_solve;    /* the factorization's leaf solves: 0 one wave per column chunk,
                              1 blocked, 2 factor shared per workgroup */
+  /* the last iterative solve: GMRES iterations, the final preconditioned
+   * relative residual ||M^-1 (b - A x)|| / ||M^-1 b|| it stopped on, and the
+   * true one ||b - A x|| / ||b|| (largest column, FP64) */
+  int iter_iterations;
+  double iter_precond_residual;
+  double iter_residual;
 } hodlr_stats;
 
 /* One node of a user-defined partition: user_IndexTree::node
@@ -291,6 +297,34 @@ int hodlr_solve_lu(hodlr_handle h, const double* B, int ldb, int nrhs, double* X
 /* recSM/recLU solve with device pointers (which = HODLR_FACTOR_SM / _LU). */
 int hodlr_solve_device(hodlr_handle h, int which, const double* dB, int ldb, int nrhs,
                        double* dX, int ldx);
+/* iterative_Solve(input_RHS, maxIterations, stop_tolerance, init_LRTolerance,
+ * input_LRMethod, directSolve_Method) (HODLR_Matrix.hpp:219,
+ * HODLR_Matrix.cpp:1138-1213): sets the handle's tolerance to
+ * init_lr_tolerance and its method to lr_method (kept afterwards, as in the
+ * reference), compresses and factors (direct_method HODLR_FACTOR_LU, the
+ * reference's "recLU", or HODLR_FACTOR_SM), then runs GMRES with restart 32
+ * on the exact operator -- the dense matrix, or for kernel handles the
+ * kernel evaluated on the device -- left-preconditioned by that HODLR solve
+ * (IML/include/gmres.h:80-151), from x0 = M^-1 b, until every column's
+ * ||M^-1 (b - A x)|| / ||M^-1 b|| is below stop_tolerance.  The reference
+ * ignores its maxIterations and stop_tolerance arguments and always uses
+ * 1000 and 1e-10; here they are honoured (pass 1000 and 1e-10 for its
+ * behaviour).  A solve that ends above the tolerance returns
+ * HODLR_ERR_NOT_CONVERGED with its last iterate in X (the reference returns
+ * the iterate without a status).  Host pointers; _device takes device
+ * pointers. */
+int hodlr_iterative_solve(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx,
+                          int max_iterations, double stop_tolerance, double init_lr_tolerance,
+                          int lr_method, int direct_method);
+int hodlr_iterative_solve_device(hodlr_handle h, const double* dB, int ldb, int nrhs, double* dX,
+                                 int ldx, int max_iterations, double stop_tolerance,
+                                 double init_lr_tolerance, int lr_method, int direct_method);
+/* Y = A X with the exact operator the iterative solver uses (the dense
+ * matrix, or the kernel evaluated on the fly), device pointers.  No
+ * reference counterpart (kernelMatrix::operator*, src/kernel.cpp:27-35, on
+ * the host). */
+int hodlr_exact_apply_device(hodlr_handle h, const double* dX, int ldx, int nrhs, double* dY,
+                             int ldy);
 /* logAbsDeterminant (HODLR_Matrix.hpp:249, HODLR_Matrix.cpp:1666-1697). */
 int hodlr_logabsdet(hodlr_handle h, double* out);
 /* determinant (HODLR_Matrix.hpp:248): the product of the LU diagonals without

@@ -1260,6 +1260,146 @@ int orc_refined_solve(void* p, const double* B, int nrhs, double* X, double targ
   return steps;
 }
 
+/* iterative_Solve (HODLR_Matrix.cpp:1138-1213) with IML++'s GMRES
+ * (IML/include/gmres.h:80-151) restated for one right-hand side at a time:
+ * tolerance init_tol and method set first, x0 = M^-1 b with M the recLU
+ * (direct 1) or recSM (0) solve, restart 32, modified Gram-Schmidt, Givens
+ * rotations, stopping when |s_(i+1)| / ||M^-1 b|| < tol inside a cycle or
+ * ||M^-1 (b - A x)|| / ||M^-1 b|| < tol after one; A is the exact operator
+ * (the dense matrix or the kernel's entries).  Returns the iterations used
+ * (of the last column) or -1 when a column did not converge; res_out gets
+ * the largest final preconditioned relative residual. */
+int orc_iterative_solve(void* p, const double* B, int nrhs, double* X, int max_iter, double tol,
+                        double init_tol, int method, int direct, double* res_out) {
+  Hodlr* h = (Hodlr*)p;
+  orc_set_tolerance(p, init_tol);
+  orc_set_lr_method(p, method);
+  h->compress();
+  if (direct == 1)
+    h->luFactorTop();
+  else
+    h->smFactor();
+  const int n = h->n, m = 32;
+  auto Msolve = [&](const std::vector<double>& v) {
+    Mat R(n, 1);
+    std::copy(v.begin(), v.end(), R.a.begin());
+    if (direct == 1) {
+      R = h->luSolve(R, h->root);
+    } else {
+      h->smSolve(h->root, R);
+    }
+    return R.a;
+  };
+  auto Aapply = [&](const std::vector<double>& v) {
+    std::vector<double> y(n, 0.0);
+    for (int j = 0; j < n; j++) {
+      const double vj = v[j];
+      for (int i = 0; i < n; i++) y[i] += h->src(i, j) * vj;
+    }
+    return y;
+  };
+  auto norm = [&](const std::vector<double>& v) {
+    double s2 = 0;
+    for (double e : v) s2 += e * e;
+    return std::sqrt(s2);
+  };
+  int status = 0, last_iter = 0;
+  double worst = 0;
+  for (int c = 0; c < nrhs; c++) {
+    std::vector<double> b(B + (size_t)c * n, B + (size_t)(c + 1) * n);
+    std::vector<double> x = Msolve(b);  // HODLR_Matrix.cpp:1175-1186
+    auto presid = [&]() {
+      std::vector<double> Ax = Aapply(x), r(n);
+      for (int i = 0; i < n; i++) r[i] = b[i] - Ax[i];
+      return Msolve(r);
+    };
+    double normb = norm(Msolve(b));  // gmres.h:89
+    std::vector<double> r = presid();
+    double beta = norm(r), resid;
+    if (normb == 0.0) normb = 1;
+    int j = 1, it = 0;
+    bool conv = false;
+    if ((resid = norm(r) / normb) <= tol) conv = true;
+    std::vector<std::vector<double>> v(m + 1);
+    std::vector<double> H((size_t)(m + 1) * m), s(m + 1), cs(m + 1), sn(m + 1);
+    auto Hm = [&](int r_, int c_) -> double& { return H[(size_t)r_ + (size_t)c_ * (m + 1)]; };
+    auto rot = [](double& dx, double& dy, double c_, double s_) {
+      const double t = c_ * dx + s_ * dy;
+      dy = -s_ * dx + c_ * dy;
+      dx = t;
+    };
+    auto update = [&](int k) {  // Update(x, k, H, s, v), gmres.h:53-69
+      std::vector<double> y(s.begin(), s.begin() + k + 1);
+      for (int i = k; i >= 0; i--) {
+        y[i] /= Hm(i, i);
+        for (int q = i - 1; q >= 0; q--) y[q] -= Hm(q, i) * y[i];
+      }
+      for (int q = 0; q <= k; q++)
+        for (int e = 0; e < n; e++) x[e] += v[q][e] * y[q];
+    };
+    while (!conv && j <= max_iter) {
+      v[0].assign(n, 0.0);
+      for (int e = 0; e < n; e++) v[0][e] = r[e] * (1.0 / beta);
+      std::fill(s.begin(), s.end(), 0.0);
+      s[0] = beta;
+      int i;
+      for (i = 0; i < m && j <= max_iter; i++, j++) {
+        std::vector<double> w = Msolve(Aapply(v[i]));
+        for (int k = 0; k <= i; k++) {
+          double d = 0;
+          for (int e = 0; e < n; e++) d += w[e] * v[k][e];
+          Hm(k, i) = d;
+          for (int e = 0; e < n; e++) w[e] -= d * v[k][e];
+        }
+        Hm(i + 1, i) = norm(w);
+        v[i + 1].assign(n, 0.0);
+        for (int e = 0; e < n; e++) v[i + 1][e] = w[e] * (1.0 / Hm(i + 1, i));
+        for (int k = 0; k < i; k++) rot(Hm(k, i), Hm(k + 1, i), cs[k], sn[k]);
+        {  // GeneratePlaneRotation (gmres.h:28-43)
+          const double dx = Hm(i, i), dy = Hm(i + 1, i);
+          if (dy == 0.0) {
+            cs[i] = 1.0;
+            sn[i] = 0.0;
+          } else if (std::fabs(dy) > std::fabs(dx)) {
+            const double t = dx / dy;
+            sn[i] = 1.0 / std::sqrt(1.0 + t * t);
+            cs[i] = t * sn[i];
+          } else {
+            const double t = dy / dx;
+            cs[i] = 1.0 / std::sqrt(1.0 + t * t);
+            sn[i] = t * cs[i];
+          }
+        }
+        rot(Hm(i, i), Hm(i + 1, i), cs[i], sn[i]);
+        rot(s[i], s[i + 1], cs[i], sn[i]);
+        if ((resid = std::fabs(s[i + 1]) / normb) < tol) {
+          update(i);
+          it = j;
+          conv = true;
+          break;
+        }
+      }
+      if (conv) break;
+      update(m - 1);
+      r = presid();
+      beta = norm(r);
+      if ((resid = beta / normb) < tol) {
+        it = j;
+        conv = true;
+      }
+    }
+    if (!conv) {
+      status = -1;
+      it = j - 1;
+    }
+    last_iter = it;
+    worst = std::max(worst, resid);
+    std::copy(x.begin(), x.end(), X + (size_t)c * n);
+  }
+  if (res_out) *res_out = worst;
+  return status < 0 ? -1 : last_iter;
+}
+
 /* which: 0 compress, 1 recSM factor, 2 recSM solve, 3 recLU factor, 4 recLU solve */
 double orc_time(void* p, int which) {
   Hodlr* h = (Hodlr*)p;

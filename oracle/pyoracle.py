@@ -67,6 +67,8 @@ def lib():
         L.orc_matvec.argtypes = [_p, _dp, _i, _dp]
         L.orc_residual_dd.argtypes = [_p, _dp, _dp, _i, _dp]
         L.orc_set_leaf_inverse.argtypes = [_p, _i]
+        L.orc_iterative_solve.restype = _i
+        L.orc_iterative_solve.argtypes = [_p, _dp, _i, _dp, _i, _d, _d, _i, _i, _dp]
         L.orc_refined_solve.restype = _i
         L.orc_refined_solve.argtypes = [_p, _dp, _i, _dp, _d, _i, _dp]
         L.orc_time.restype = _d
@@ -244,6 +246,18 @@ class OracleHODLR:
         Y = np.zeros_like(X, order="F")
         lib().orc_matvec(self.h, _ptr(X), X.shape[1], _ptr(Y))
         return Y
+
+    def iterative_Solve(self, B, maxIterations=1000, stop_tolerance=1e-10, init_LRTolerance=1e-2,
+                        method="partialPiv_ACA", direct="recLU"):
+        """(X, iterations, final preconditioned residual); iterations -1: not converged."""
+        B = np.asfortranarray(B.reshape(self.n, -1), dtype=np.float64)
+        X = np.zeros_like(B, order="F")
+        res = np.zeros(1)
+        it = lib().orc_iterative_solve(self.h, _ptr(B), B.shape[1], _ptr(X), int(maxIterations),
+                                       float(stop_tolerance), float(init_LRTolerance),
+                                       {"partialPiv_ACA": 0, "SVD": 1}[method],
+                                       {"recSM": 0, "recLU": 1}[direct], _ptr(res))
+        return X, int(it), float(res[0])
 
     def set_leaf_inverse(self, on=True):
         """Measurement variant, not the reference: recSM's leaf solves multiply

@@ -385,3 +385,29 @@ def test_oracle_refinement_converges_to_rounding(oracle):
     assert steps == 6
     assert e0 > 1e-9, e0  # the unrefined solution is far off here
     assert e1 < 1e-14, (e1, e0)
+
+
+def test_oracle_iterative_solve_radial_imq(oracle):
+    """iterative_Solve_Test (test/test.cpp:347-360) at n = 2000: the IMQ
+    radial matrix on [-1, 1] with a zero diagonal, leaf 30, compressed at
+    1e-2, GMRES with the recLU preconditioner to 1e-10: relative error
+    against the exact solution below the reference's 1e-6; the
+    preconditioned residual below the tolerance; the same solution as a
+    dense LU solve to the conditioning."""
+    n = 2000
+    pts = np.linspace(-1, 1, n).reshape(-1, 1)
+    A = oracle.kernel_block(pts, "imq", 1.0, 0.0, 0, 0, n, n, diag_mode=1)
+    x = I.linspace_m2_2(n)
+    b = A @ x
+    o = oracle.OracleHODLR(n, leaf=30, dense=A)
+    X, it, res = o.iterative_Solve(b, 1000, 1e-10, 1e-2, "partialPiv_ACA", "recLU")
+    xd = np.linalg.solve(A, b)
+    print(f"iterations {it}, preconditioned residual {res:.2e}, error "
+          f"{np.linalg.norm(X[:, 0] - x) / np.linalg.norm(x):.2e}")
+    assert it > 0 and res < 1e-10
+    assert np.linalg.norm(X[:, 0] - x) / np.linalg.norm(x) < 1e-6
+    assert np.linalg.norm(X[:, 0] - xd) / np.linalg.norm(xd) < 1e-8
+    # a direct recLU solve at the loose tolerance is far less accurate
+    o2 = oracle.OracleHODLR(n, leaf=30, dense=A)
+    o2.set_LRTolerance(1e-2)
+    assert np.linalg.norm(o2.recLU_Solve(b)[:, 0] - x) / np.linalg.norm(x) > 1e-4
