@@ -37,6 +37,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "factor+solve time at N=256K to 1e-10 rel. residual: mixed+learned vs plain FP64"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFS = 78.6
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "aca_gauss2d_262144_traffic.txt")
 
 
@@ -57,7 +58,14 @@ def parse():
                    help="fix the launch shapes instead of autotuning: ACA rows per thread, "
                         "GEMM tile rows, split-K target, LU panel, MFMA, leaf-solve variant "
                         "(e.g. 4,0,4096,32,1,0)")
-    p.add_argument("--cpu-n", type=int, nargs=2, default=[4096, 8192])
+    p.add_argument("--cpu-n", type=int, nargs="+", default=[2048, 4096, 8192, 16384],
+                   help="oracle sample sizes (at least 2; the largest is the reported sample)")
+    p.add_argument("--no-iterative", action="store_true",
+                   help="skip the GMRES solve against the exact kernel")
+    p.add_argument("--profile-factor", action="store_true",
+                   help="only compress the workload, then run one FP64 recSM factorization "
+                        "with its kernel groups timed, print them and exit (for counter runs: "
+                        "every GEMM, LU and solve kernel of the process is the factorization's)")
     p.add_argument("--workload", default="gauss2d_262144", choices=["gauss2d_262144", "gp_batch_512"],
                    help="gp_batch_512: the batched Gaussian-process systems instead of the "
                         "headline system (a separate JSON line, not the headline metric)")
@@ -192,10 +200,45 @@ def gp_batch(a):
     print(json.dumps(out), flush=True)
 
 
+def work_counts(nodes, n, nrhs):
+    """Algorithmic work of the recSM path on a tree with these nodes
+    (level, min_i, split, max_i, rank_top, rank_bottom), ranges not among
+    them being leaves:
+      aca_mac  sum over blocks of (m+n) k(k+1)/2  (residual multiply-adds),
+      aca_ent  sum over blocks of (m+n)(k+1)      (kernel entries evaluated),
+      factor   flops of the leaf LUs, the leaf and node solves of the
+               ancestors' U^SM columns, the V^T U^SM products, the node
+               matrices' LUs and the U^SM updates,
+      solve    flops of one solve with nrhs columns."""
+    rng = {(int(r[1]), int(r[3])): (int(r[2]), int(r[4]), int(r[5])) for r in nodes}
+    w = dict(aca_mac=0.0, aca_ent=0.0, factor=0.0, solve=0.0)
+    stack = [(0, n - 1, 0)]
+    while stack:
+        lo, hi, anc = stack.pop()
+        if (lo, hi) not in rng:  # a leaf
+            s_ = hi - lo + 1
+            w["factor"] += 2.0 / 3.0 * s_ ** 3 + 2.0 * s_ * s_ * anc
+            w["solve"] += 2.0 * s_ * s_ * nrhs
+            continue
+        sp, kt, kb = rng[(lo, hi)]
+        st, sb, K = sp - lo + 1, hi - sp, kt + kb
+        for m, nn, k in ((st, sb, kt), (sb, st, kb)):
+            w["aca_mac"] += (m + nn) * k * (k + 1) / 2.0
+            w["aca_ent"] += (m + nn) * (k + 1.0)
+        prod = kb * st + kt * sb      # the V^T U^SM products' rows x ranks
+        upd = kt * st + kb * sb       # the updates' rows x ranks
+        w["factor"] += (2.0 * prod * (anc + K) + 2.0 / 3.0 * K ** 3 + 2.0 * K * K * anc
+                        + 2.0 * upd * anc)
+        w["solve"] += nrhs * (2.0 * prod + 2.0 * K * K + 2.0 * upd)
+        stack.append((lo, sp, anc + kt))
+        stack.append((sp + 1, hi, anc + kb))
+    return w
+
+
 def cpu_baseline(sizes, nrhs):
-    """The oracle's recSM (compress + factor + solve, one thread) at two
-    sample sizes of the same system; the larger is the reported sample, the
-    pair gives the exponent of a power-law extrapolation to the workload."""
+    """The oracle's recSM (compress + factor + solve, one thread) at the
+    sample sizes of the same system, with each run's phase times and work
+    counts; the largest is the reported sample."""
     from oracle import pyoracle
     res = []
     for n in sizes:
@@ -205,11 +248,36 @@ def cpu_baseline(sizes, nrhs):
         o.set_LRTolerance(1e-10)
         t0 = time.perf_counter()
         o.recSM_Solve(B)
-        res.append((n, time.perf_counter() - t0, o.times()))
+        wall = time.perf_counter() - t0
+        res.append(dict(n=n, wall=wall, phases=o.times(), work=work_counts(o.nodes(), n, nrhs)))
+        del o
     return res
 
 
+def fit_cpu(runs):
+    """Least-squares fit of the oracle's phase times to its work counts:
+    compress = a aca_mac + b aca_ent (non-negative), factor = c factor,
+    solve = d solve.  Returns the coefficients and a predictor."""
+    from scipy.optimize import nnls
+    Ac = np.array([[r["work"]["aca_mac"], r["work"]["aca_ent"]] for r in runs])
+    tc = np.array([r["phases"]["compress"] for r in runs])
+    sc = Ac.max(axis=0)
+    ab, _ = nnls(Ac / sc, tc)
+    ab = ab / sc
+    coef = {"compress_per_mac": ab[0], "compress_per_entry": ab[1]}
+    for ph, key in (("sm_factor", "factor"), ("sm_solve", "solve")):
+        x = np.array([r["work"][key] for r in runs])
+        t = np.array([r["phases"][ph] for r in runs])
+        coef[key + "_per_flop"] = float(x @ t / (x @ x))
+
+    def predict(w):
+        return (coef["compress_per_mac"] * w["aca_mac"] + coef["compress_per_entry"] * w["aca_ent"]
+                + coef["factor_per_flop"] * w["factor"] + coef["solve_per_flop"] * w["solve"])
+    return {k: float(v) for k, v in coef.items()}, predict
+
+
 def main():
+    global SHAPES
     a = parse()
     if a.copy_kernel:
         copy_kernel()
@@ -220,6 +288,18 @@ def main():
         sys.exit(2)
     if a.workload == "gp_batch_512":
         gp_batch(a)
+        return
+    if a.profile_factor:
+        SHAPES = [int(v) for v in a.launch_shapes.split(",")] if a.launch_shapes else None
+        pts, _ = workload(a.n, 1)
+        g = make_handle(pts, a.target, "fp64", 0)
+        g.storeLRinTree()
+        g.set_profiling(True)
+        g.recSM_Factorize()
+        st = g.stats()
+        print(json.dumps({"workload": f"gauss2d_kd N={a.n} recSM factorization",
+                          "factor_s": st["factor_time"], "groups": st["factor_groups"],
+                          "max_rank": st["max_rank"]}), flush=True)
         return
     import torch
 
@@ -242,7 +322,6 @@ def main():
     # untimed: the autotuner picks launch shapes for this problem shape; every
     # later handle of the shape (both paths below) uses them
     tuning = None
-    global SHAPES
     shapes = [int(v) for v in a.launch_shapes.split(",")] if a.launch_shapes else None
     SHAPES = shapes
     if shapes is not None:
@@ -274,6 +353,7 @@ def main():
     bn = np.linalg.norm(B, axis=0)
     res = float(np.max(np.linalg.norm(g.residual(B, X), axis=0) / bn))
     res64 = float(np.max(np.linalg.norm(B - g.matvec(X), axis=0) / bn))
+    nodes_wl = g.nodes()
     g = None
     spot = kernel_spot_check(pts, B, X)
 
@@ -297,6 +377,10 @@ def main():
                            "double-double residuals (flexible GMRES when a step gains less "
                            "than 8x; the FP32 factorization does not converge at this "
                            "condition number: see mixed)"},
+        "timed_path": "FP64-preconditioned refinement to 1e-10 with learned panel sizing (the "
+                      "metric's mixed leg does not converge here: see mixed; plain_fp64 is the "
+                      "same refined path without the predictor; fp64_unrefined is the "
+                      "reference's plain recSM_Solve, which stops near 1e-6)",
         "phases_s": {"compress": st["compress_time"], "factor": st["factor_time"],
                      "solve_and_refine": st["solve_time"]},
         "refine_steps": st["refine_iterations"],
@@ -312,7 +396,8 @@ def main():
         "predictor": {"blocks": st["pred_blocks"], "misses": st["pred_misses"],
                       "mean_abs_log_rank_error": st["pred_log_err"]},
         "step_walls_s": walls,
-        "plain_fp64": {"value": plain, "step_walls_s": pwalls, "compress_s": pst["compress_time"],
+        "plain_fp64": {"label": "refined, no predictor",
+                       "value": plain, "step_walls_s": pwalls, "compress_s": pst["compress_time"],
                        "factor_s": pst["factor_time"], "solve_and_refine_s": pst["solve_time"],
                        "residual": pst["last_residual"]},
         "speedup_vs_plain": plain / value,
@@ -320,6 +405,20 @@ def main():
     }
     from dense_hodlr_amd import memory_stats
     out["device_memory_cache"] = memory_stats()
+
+    # the reference's own path: plain FP64 recSM_Solve, no refinement
+    uwalls = []
+    for _ in range(a.steps):
+        g = make_handle(pts, a.target, "fp64", 0)
+        uwalls.append(solve_device(g, dB.data_ptr(), dX.data_ptr(), n, nrhs))
+        su = g.stats()
+        Xu = dX.cpu().numpy().T.copy(order="F")
+        ru = float(np.max(np.linalg.norm(g.residual(B, Xu), axis=0) / bn))
+        g = None
+    out["fp64_unrefined"] = {"label": "plain recSM_Solve (no refinement, no predictor)",
+                             "value": float(np.median(uwalls)), "step_walls_s": uwalls,
+                             "compress_s": su["compress_time"], "factor_s": su["factor_time"],
+                             "solve_s": su["solve_time"], "residual": ru}
 
     if not a.no_mixed:
         from dense_hodlr_amd import HodlrError, HodlrNotConverged
@@ -385,20 +484,103 @@ def main():
                               "col": 1e6 * sp["aca_col_time"] / iters},
             "launches": iters,
         }
+        # the factorization of the same handle, its kernel groups timed with
+        # HIP events on the streams they run on (FP64: 78.6 TF/s on the
+        # vector units and on the matrix cores alike)
+        g.recSM_Factorize()
+        fg = g.stats()["factor_groups"]
+        names = {"gemm": "gemm_mfma_f64_kernel (grouped: V^T U^SM products and U^SM updates of "
+                         "every node of a level)",
+                 "lu": "batched LU (lu_panel_reg_kernel + lu_step_kernel + trailing GEMM), "
+                       "leaves and node matrices",
+                 "leaf_solve": "lu_solve_wave_kernel: leaf LU solves of the ancestors' U^SM "
+                               "columns",
+                 "node_solve": "node-matrix LU solves (lu_solve_wave/blk kernels)"}
+        more = []
+        for key, label in names.items():
+            q = fg[key]
+            if q["time"] <= 0:
+                continue
+            tf = q["flops"] / q["time"] / 1e12
+            more.append({"kernel": label, "bound": "mfma" if key == "gemm" else "fp64",
+                         "achieved": tf, "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                         "frac": tf / FP64_PEAK_TFS, "time_s": q["time"], "flops": q["flops"],
+                         "bytes": q["bytes"], "launches": q["launches"],
+                         "roofline_time_s": q["bound_time"],
+                         "frac_of_roofline_time": q["bound_time"] / q["time"]})
+        out["roofline_factorization"] = more
         del g
 
+    if not a.no_iterative:
+        # GMRES against the exact kernel (evaluated on the device), left
+        # preconditioned by the FP64 recSM solve at the same ACA tolerance:
+        # the solution's residual against K itself, which the direct and
+        # refined solves above only reach to the compression's distance
+        from dense_hodlr_amd import HODLR_Matrix, HodlrNotConverged, _lib
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+        if SHAPES:
+            g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 0)
+        t0 = time.perf_counter()
+        status = "converged"
+        try:
+            _lib.check(_lib.lib().hodlr_iterative_solve_device(
+                g._h, C.c_void_p(dB.data_ptr()), n, nrhs, C.c_void_p(dX.data_ptr()), n, 50,
+                1e-10, 1e-10, _lib.LR_METHODS["partialPiv_ACA"], _lib.FACTOR_SM), g._h)
+        except HodlrNotConverged:
+            status = "HODLR_ERR_NOT_CONVERGED"
+        wall = time.perf_counter() - t0
+        si = g.stats()
+        Xi = dX.cpu().numpy().T.copy(order="F")
+        out["iterative_exact"] = {
+            "label": "iterative_Solve: GMRES(32) on the exact kernel, recSM preconditioner at "
+                     "ACA tolerance 1e-10, stop at preconditioned residual 1e-10",
+            "status": status, "wall_s": wall, "iterations": si["iter_iterations"],
+            "preconditioned_residual": si["iter_precond_residual"],
+            "residual_vs_kernel": si["iter_residual"],
+            "kernel_spot_check": kernel_spot_check(pts, B, Xi)}
+        g = None
+
+    gpu_sample = None
     if not a.no_cpu_baseline:
-        pts_ = cpu_baseline(a.cpu_n, nrhs)
-        (n0, t0, _), (n1, t1, parts) = pts_[0], pts_[-1]
-        p = math.log(t1 / t0) / math.log(n1 / n0) if t1 > 0 and t0 > 0 and n1 != n0 else 2.0
+        # the GPU on the CPU sample itself: the same plain recSM_Solve
+        ns = max(a.cpu_n)
+        spts, sB = workload(ns, nrhs, seed=777)
+        sdB = torch.from_numpy(np.ascontiguousarray(sB.T)).to("cuda")
+        sdX = torch.empty_like(sdB)
+        sw = []
+        for _ in range(3):
+            from dense_hodlr_amd import HODLR_Matrix
+            g = HODLR_Matrix.from_kernel(spts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+            g.set_LRTolerance(1e-10)
+            torch.cuda.synchronize()
+            sw.append(solve_device(g, sdB.data_ptr(), sdX.data_ptr(), ns, nrhs))
+            g = None
+        gpu_sample = {"n": ns, "value_s": min(sw), "path": "plain recSM_Solve on the GPU"}
+    if not a.no_cpu_baseline:
+        runs = cpu_baseline(sorted(a.cpu_n), nrhs)
+        last = runs[-1]
+        coef, predict = fit_cpu(runs)
+        fit_err = max(abs(predict(r["work"]) - r["wall"]) / r["wall"] for r in runs)
+        hold = None
+        if len(runs) >= 3:
+            _, pred_h = fit_cpu(runs[:-1])
+            hold = {"fit_sizes": [r["n"] for r in runs[:-1]], "predicted_n": last["n"],
+                    "predicted_s": pred_h(last["work"]), "measured_s": last["wall"]}
+        wl = work_counts(nodes_wl, n, nrhs)
         out["cpu_baseline"] = {
-            "value": t1, "unit": "s", "cores": 1, "kind": "port",
+            "value": last["wall"], "unit": "s", "cores": 1, "kind": "port",
             "sample": f"oracle recSM compress+factor+solve (no refinement), same kernel, leaf, "
-                      f"tolerance and {nrhs} RHS at N={n1}, one thread",
-            "phases_s": parts,
-            "sizes_s": {str(k): v for k, v, _ in pts_},
-            "extrapolated_to_workload_s": t1 * (n / n1) ** p,
-            "extrapolation_exponent": p,
+                      f"tolerance and {nrhs} RHS at N={last['n']}, one thread",
+            "phases_s": last["phases"],
+            "sizes_s": {str(r["n"]): r["wall"] for r in runs},
+            "gpu_same_sample": gpu_sample,
+            "workload_estimate_s": predict(wl),
+            "workload_estimate_kind": "fit, not measured: the oracle's phase times fitted to "
+                                      "its algorithmic work counts at the sample sizes, "
+                                      "evaluated on the workload's own ranks",
+            "fit": {"model": "compress = a sum (m+n)k(k+1)/2 + b sum (m+n)(k+1); factor = c "
+                             "flops; solve = d flops", "coefficients": coef,
+                    "max_rel_residual": fit_err, "holdout": hold},
         }
     print(json.dumps(out), flush=True)
 

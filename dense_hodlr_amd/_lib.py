@@ -95,6 +95,11 @@ class Stats(C.Structure):
         ("iter_iterations", C.c_int),
         ("iter_precond_residual", C.c_double),
         ("iter_residual", C.c_double),
+        ("fac_time", C.c_double * 4),
+        ("fac_flops", C.c_double * 4),
+        ("fac_bytes", C.c_double * 4),
+        ("fac_bound_time", C.c_double * 4),
+        ("fac_launches", C.c_int * 4),
     ]
 
 

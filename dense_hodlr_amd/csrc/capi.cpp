@@ -737,6 +737,13 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
     out->aca_col_time = s.aca_col_time;
     out->aca_launches = s.aca_launches;
     out->aca_bytes = s.aca_bytes;
+    for (int c = 0; c < 4; c++) {
+      out->fac_time[c] = s.fac_time[c];
+      out->fac_flops[c] = s.fac_flops[c];
+      out->fac_bytes[c] = s.fac_bytes[c];
+      out->fac_bound_time[c] = s.fac_bound[c];
+      out->fac_launches[c] = s.fac_launches[c];
+    }
   });
 }
 

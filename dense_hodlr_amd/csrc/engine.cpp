@@ -160,6 +160,7 @@ Engine::~Engine() {
   if (aux_) (void)hipStreamSynchronize(aux_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (auto e : prof_ev_) (void)hipEventDestroy(e);
+  for (auto e : span_events_) (void)hipEventDestroy(e);
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev_cap_) (void)hipEventDestroy(ev_cap_);
@@ -484,6 +485,73 @@ void Engine::grow(DBuf<double>& b, size_t doubles) {
     b.alloc(doubles);
     buf_gen_++;
   }
+}
+
+/* ------------------------------------------------------------------ */
+/* profiling of the factorization's kernel groups                      */
+/* ------------------------------------------------------------------ */
+namespace {
+constexpr double FP64_PEAK = 78.6e12;  // MI355X dense FP64, vector and matrix cores
+constexpr double HBM_PEAK = 8.0e12;
+
+template <class T>
+void gemm_work(const std::vector<GemmDescT<T>>& d, double& fl, double& by) {
+  fl = by = 0;
+  for (auto& g : d) {
+    fl += 2.0 * g.M * (double)g.N * g.K;
+    by += sizeof(T) * ((double)g.M * g.K + (double)g.K * g.N +
+                       (double)g.M * g.N * (g.beta != T(0) ? 2 : 1));
+  }
+}
+
+void lu_work(const std::vector<LuDesc>& d, double& fl, double& by) {
+  fl = by = 0;
+  for (auto& l : d) {
+    fl += 2.0 / 3.0 * (double)l.n * l.n * l.n;
+    by += 16.0 * (double)l.n * l.n;
+  }
+}
+}  // namespace
+
+int Engine::prof_begin(hipStream_t s) {
+  if (!prof_fac_) return -1;
+  if (prof_depth_++ > 0) return -1;  // outermost group only
+  while (span_next_ + 2 > span_events_.size()) {
+    hipEvent_t e;
+    HODLR_HIP_CHECK(hipEventCreate(&e));
+    span_events_.push_back(e);
+  }
+  const int tok = (int)span_next_;
+  span_next_ += 2;
+  HODLR_HIP_CHECK(hipEventRecord(span_events_[tok], s));
+  return tok;
+}
+
+void Engine::prof_end(int tok, hipStream_t s, int cat, double flops, double bytes) {
+  if (!prof_fac_) return;
+  prof_depth_--;
+  if (tok < 0) return;
+  HODLR_HIP_CHECK(hipEventRecord(span_events_[tok + 1], s));
+  spans_.push_back({cat, flops, bytes, span_events_[tok], span_events_[tok + 1]});
+}
+
+/* after both streams are idle */
+void Engine::prof_collect() {
+  for (int c = 0; c < 4; c++) {
+    stats_.fac_time[c] = stats_.fac_flops[c] = stats_.fac_bytes[c] = stats_.fac_bound[c] = 0;
+    stats_.fac_launches[c] = 0;
+  }
+  for (auto& sp : spans_) {
+    float ms = 0;
+    HODLR_HIP_CHECK(hipEventElapsedTime(&ms, sp.a, sp.b));
+    stats_.fac_time[sp.cat] += ms * 1e-3;
+    stats_.fac_flops[sp.cat] += sp.flops;
+    stats_.fac_bytes[sp.cat] += sp.bytes;
+    stats_.fac_bound[sp.cat] += std::max(sp.flops / FP64_PEAK, sp.bytes / HBM_PEAK);
+    stats_.fac_launches[sp.cat]++;
+  }
+  spans_.clear();
+  span_next_ = 0;
 }
 
 template <class T>
@@ -1273,7 +1341,11 @@ void Engine::factor_leaves() {
   HODLR_HIP_CHECK(hipMemcpyAsync(Lfac_.p, Lorig_.p, leaf_elems_ * sizeof(double), hipMemcpyDeviceToDevice, stream_));
   std::vector<LuDesc> ld;
   for (auto& lf : leaves_) ld.push_back({Lfac_.p + lf.mat_off, Lpiv_.p + lf.piv_off, lf.s, lf.s});
+  double fl = 0, by = 0;
+  lu_work(ld, fl, by);
+  const int t = prof_begin(stream_);
   lu_batch(ld);
+  prof_end(t, stream_, FAC_LU, fl, by);
   leaves_factored_ = true;
 }
 
@@ -1324,6 +1396,10 @@ void Engine::factorize_fp64(int which) {
   finish_root();  // a previous factorization's root LU may still use F's buffers
   buf_gen_++;  // captured solves refer to the factorization's buffers
   factor_time_pending_ = false;
+  prof_fac_ = profiling_;
+  prof_depth_ = 0;
+  spans_.clear();
+  span_next_ = 0;
   HODLR_HIP_CHECK(hipEventRecord(ev_f0_, stream_));
   factor_leaves();
   const size_t N = (size_t)ntot_;
@@ -1353,7 +1429,17 @@ void Engine::factorize_fp64(int which) {
     // leaves of level d: W[rows, 0:off_d] <- A_leaf^-1 W[rows, 0:off_d] (in
     // place by the LU factors; the columns of level d and deeper are zero
     // padding in a leaf's rows)
-    if (offd > 0) leaf_solves(level_leaves_[d], W, ld, offd, false);
+    if (offd > 0 && !level_leaves_[d].empty()) {
+      double fl = 0, by = 0;
+      for (int li : level_leaves_[d]) {
+        const double sz = leaves_[li].s;
+        fl += 2.0 * sz * sz * offd;
+        by += 8.0 * (sz * sz + 2.0 * sz * offd);
+      }
+      const int t = prof_begin(stream_);
+      leaf_solves(level_leaves_[d], W, ld, offd, false);
+      prof_end(t, stream_, FAC_LEAF_SOLVE, fl, by);
+    }
     if (level_nodes_[d].empty()) continue;
     // nodes of level d
     const int ncols = offd + K_[d];
@@ -1416,24 +1502,43 @@ void Engine::factorize_fp64(int which) {
                                ld, -1.0, 1.0));
       }
     }
+    double fl = 0, by = 0;
+    gemm_work(gz, fl, by);
+    int t = prof_begin(stream_);
     run_gemm(gz);
+    prof_end(t, stream_, FAC_GEMM, fl, by);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
+    lu_work(lu, fl, by);
     if (offd == 0 && overlap_root_) {
       // the roots: nothing later in the factorization reads their LU, so it
       // runs on aux_ and the solve that follows waits for it at its last level
       HODLR_HIP_CHECK(hipEventRecord(ev_cap_, stream_));
       HODLR_HIP_CHECK(hipStreamWaitEvent(aux_, ev_cap_, 0));
+      t = prof_begin(aux_);
       lu_batch(lu, aux_, aux_arena_);
+      prof_end(t, aux_, FAC_LU, fl, by);
       HODLR_HIP_CHECK(hipEventRecord(ev_root_, aux_));
       HODLR_HIP_CHECK(hipEventRecord(ev_froot_, aux_));
       F.root_pending = true;
       factor_time_pending_ = true;
     } else {
+      t = prof_begin(stream_);
       lu_batch(lu);
+      prof_end(t, stream_, FAC_LU, fl, by);
     }
     if (offd > 0) {
+      fl = by = 0;
+      for (auto& v : sd) {
+        fl += 2.0 * v.n * (double)v.n * v.ncols;
+        by += 8.0 * ((double)v.n * v.n + 2.0 * v.n * (double)v.ncols);
+      }
+      t = prof_begin(stream_);
       lu_solve_batch(sd);
+      prof_end(t, stream_, FAC_NODE_SOLVE, fl, by);
+      gemm_work(gu, fl, by);
+      t = prof_begin(stream_);
       run_gemm(gu);
+      prof_end(t, stream_, FAC_GEMM, fl, by);
     }
   }
   HODLR_HIP_CHECK(hipEventRecord(ev_f1_, stream_));
@@ -1442,6 +1547,11 @@ void Engine::factorize_fp64(int which) {
   HODLR_HIP_CHECK(hipEventElapsedTime(&ms, ev_f0_, ev_f1_));
   stats_.factor_stream_time = ms * 1e-3;
   stats_.factor_time = ms * 1e-3;  // completed by sync_stats while the roots' LU is pending
+  if (prof_fac_) {
+    HODLR_HIP_CHECK(hipStreamSynchronize(aux_));
+    prof_collect();
+    prof_fac_ = false;
+  }
   arena_.reset();
   F.done = true;
 }

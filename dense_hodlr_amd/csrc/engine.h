@@ -300,7 +300,15 @@ class Engine {
     double iter_residual = 0;          // and ||b - A x|| / ||b|| (largest column)
     int pred_blocks = 0, pred_misses = 0;  // with a rank predictor attached
     double pred_log_err = 0;               // mean |log predicted - log ACA rank|
+    // with profiling on, the last FP64 factorization by kernel group
+    // (FAC_GEMM, FAC_LU, FAC_LEAF_SOLVE, FAC_NODE_SOLVE): summed HIP-event
+    // durations, algorithmic flops and bytes, the sum over launches of
+    // max(flops / FP64 peak, bytes / HBM peak), and the launch count
+    double fac_time[4] = {0, 0, 0, 0}, fac_flops[4] = {0, 0, 0, 0}, fac_bytes[4] = {0, 0, 0, 0},
+           fac_bound[4] = {0, 0, 0, 0};
+    int fac_launches[4] = {0, 0, 0, 0};
   };
+  enum { FAC_GEMM = 0, FAC_LU = 1, FAC_LEAF_SOLVE = 2, FAC_NODE_SOLVE = 3 };
   void set_profiling(bool on) { profiling_ = on; }
   /* Times the candidate launch shapes on this problem (compression for the
    * ACA rows per thread; FP64 factorization for the GEMM tiles, split-K
@@ -325,6 +333,21 @@ class Engine {
  private:
   void build_trees();
   void upload_sources();
+  /* Profiling of the FP64 factorization's kernel groups: a span is a pair of
+   * events around one group launch (outermost only) with its work. */
+  struct ProfSpan {
+    int cat;
+    double flops, bytes;
+    hipEvent_t a, b;
+  };
+  bool prof_fac_ = false;
+  int prof_depth_ = 0;
+  std::vector<ProfSpan> spans_;
+  std::vector<hipEvent_t> span_events_;
+  size_t span_next_ = 0;
+  int prof_begin(hipStream_t s);
+  void prof_end(int token, hipStream_t s, int cat, double flops, double bytes);
+  void prof_collect();
   void reset_attributes();
   // each compresses the blocks whose node uses its method; ranks per block
   // (2 per node: top, bottom) into blk_rank

@@ -319,6 +319,11 @@ class HODLR_Matrix:
                     graph_launches=s.graph_launches, iter_iterations=s.iter_iterations,
                     iter_precond_residual=s.iter_precond_residual,
                     iter_residual=s.iter_residual,
+                    factor_groups={
+                        name: dict(time=s.fac_time[i], flops=s.fac_flops[i],
+                                   bytes=s.fac_bytes[i], bound_time=s.fac_bound_time[i],
+                                   launches=s.fac_launches[i])
+                        for i, name in enumerate(("gemm", "lu", "leaf_solve", "node_solve"))},
                     tuning=dict(aca_rows_per_thread=s.tune_aca_rows_per_thread,
                                 gemm_tile_m=s.tune_gemm_tile_m,
                                 gemm_split_target=s.tune_gemm_split_target,

@@ -138,6 +138,20 @@ typedef struct hodlr_stats {
   int iter_iterations;
   double iter_precond_residual;
   double iter_residual;
+  /* with hodlr_set_profiling(h, 1), the last FP64 factorization by kernel
+   * group, index 0: the grouped GEMMs of the node steps (V^T U^SM products
+   * and U^SM -= U^SM' Y updates), 1: the batched LU factorizations (leaves
+   * and node matrices), 2: the leaf solves of the ancestors' U^SM columns,
+   * 3: the node matrices' solves.  Summed HIP-event durations per group
+   * launch, algorithmic flops (2 m n k per product, (2/3) n^3 per LU, 2 n^2
+   * per solved column) and bytes (operands once, an updated output read and
+   * written), the sum over launches of max(flops / 78.6 TF/s, bytes / 8 TB/s)
+   * (the time at MI355X's FP64 and HBM peaks), and the launch count */
+  double fac_time[4];
+  double fac_flops[4];
+  double fac_bytes[4];
+  double fac_bound_time[4];
+  int fac_launches[4];
 } hodlr_stats;
 
 /* One node of a user-defined partition: user_IndexTree::node
@@ -306,13 +320,17 @@ int hodlr_solve_device(hodlr_handle h, int which, const double* dB, int ldb, int
  * on the exact operator -- the dense matrix, or for kernel handles the
  * kernel evaluated on the device -- left-preconditioned by that HODLR solve
  * (IML/include/gmres.h:80-151), from x0 = M^-1 b, until every column's
- * ||M^-1 (b - A x)|| / ||M^-1 b|| is below stop_tolerance.  The reference
- * ignores its maxIterations and stop_tolerance arguments and always uses
- * 1000 and 1e-10; here they are honoured (pass 1000 and 1e-10 for its
- * behaviour).  A solve that ends above the tolerance returns
- * HODLR_ERR_NOT_CONVERGED with its last iterate in X (the reference returns
- * the iterate without a status).  Host pointers; _device takes device
- * pointers. */
+ * ||M^-1 (b - A x)|| / ||M^-1 b|| is below stop_tolerance as gmres.h
+ * decides it: by the Arnoldi estimate within a cycle (:123-129) or by the
+ * residual at a restart (:135-140).  The reference ignores its maxIterations
+ * and stop_tolerance arguments and always uses 1000 and 1e-10; here they are
+ * honoured (pass 1000 and 1e-10 for its behaviour).  A column that has not
+ * converged after max_iterations returns HODLR_ERR_NOT_CONVERGED with the
+ * last iterate in X (the reference returns the iterate without a status).
+ * hodlr_get_stats reports the iterations, the preconditioned residual
+ * evaluated at the end and the true residual ||b - A x|| / ||b|| (both can
+ * sit above a tolerance near the rounding floor of their FP64 evaluation,
+ * eps ||A|| ||x|| / ||b||).  Host pointers; _device takes device pointers. */
 int hodlr_iterative_solve(hodlr_handle h, const double* B, int ldb, int nrhs, double* X, int ldx,
                           int max_iterations, double stop_tolerance, double init_lr_tolerance,
                           int lr_method, int direct_method);

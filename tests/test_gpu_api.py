@@ -128,6 +128,34 @@ def test_factor_time_includes_root_lu():
     assert st["factor_time"] >= st["factor_stream_time"]
 
 
+def test_factorization_kernel_groups_profiled():
+    """With profiling on, the factorization's kernel groups carry their event
+    times and work: the LU flops are (2/3) n^3 over every leaf and node
+    matrix (n = kt + kb), the solves 2 n^2 per column over the leaves'
+    ancestor columns, and the grouped GEMMs' flops the node products'; the
+    groups' times fit inside the factorization's."""
+    _case("factorization kernel groups")
+    n = 16384
+    pts = I.points_2d_kd(n, 7)
+    g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+    g.set_LRTolerance(1e-10)
+    g.storeLRinTree()
+    g.set_profiling(True)
+    g.recSM_Factorize()
+    st = g.stats()
+    fg = st["factor_groups"]
+    nodes = g.nodes()
+    leaf = n // (2 ** (int(nodes[:, 0].max()) + 1))
+    K = nodes[:, 4] + nodes[:, 5]
+    lu = 2.0 / 3.0 * (float(np.sum(K.astype(float) ** 3)) + (n // leaf) * float(leaf) ** 3)
+    print({k: (v["time"], v["flops"], v["launches"]) for k, v in fg.items()}, flush=True)
+    assert abs(fg["lu"]["flops"] - lu) <= 1e-9 * lu
+    for k in ("gemm", "lu", "leaf_solve", "node_solve"):
+        assert fg[k]["time"] > 0 and fg[k]["launches"] > 0 and fg[k]["flops"] > 0, k
+        assert fg[k]["bound_time"] > 0, k
+    assert sum(v["time"] for v in fg.values()) <= 1.05 * st["factor_time"]
+
+
 def test_dense_constructor_from_binary_file(tmp_path, oracle):
     """HODLR_Matrix(readBinaryIntoMatrixXd(path), ...) as the reference's
     drivers build it (helperFunctions.cpp:370-371): the handle made from the
