@@ -47,6 +47,17 @@ struct SolveDesc {
   int ncols;
   int chunk0; /* first workgroup of this problem */
   int pad_;
+  /* the interchanges as one permutation (row i of P B is row perm[i] of B);
+   * read by the matrix-core kernel only */
+  const int* perm = nullptr;
+};
+
+/* Build job of one permutation: perm[0..n) from the interchanges piv[0..n). */
+struct PermBuild {
+  const int* piv;
+  int* perm;
+  int n;
+  int pad_;
 };
 
 /* Dense block of entries: out[i + j*ldo] = K(i0 + i, j0 + j) of instance
@@ -174,6 +185,18 @@ int lu_solve_small_wgs(int ncols, int maxn, bool wave);
  * workgroups, lu_solve_shared_wgs(ncols, maxn) per problem.  Returns false,
  * and launches nothing, when maxn exceeds the limit. */
 int lu_solve_shared_wgs(int ncols, int maxn);
+/* The same solves on the matrix cores for systems of at most
+ * LU_SOLVE_MFMA_MAX rows (SolveDesc::perm required): one wave per system and
+ * 16 columns, lu_solve_mfma_units(ncols) per problem numbered by chunk0.
+ * Not bit-identical to the substitution kernels (the off-diagonal tiles are
+ * summed four terms at a time by the matrix cores).  Returns false, and
+ * launches nothing, when maxn exceeds the limit. */
+constexpr int LU_SOLVE_MFMA_MAX = 256;
+int lu_solve_mfma_units(int ncols);
+bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s);
+/* One wave per job; n <= LU_PERM_MAX. */
+constexpr int LU_PERM_MAX = 4096;
+void launch_lu_perm(const PermBuild* d, int count, hipStream_t s);
 bool launch_lu_solve_shared(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
                            bool blocked, hipStream_t s);

@@ -885,12 +885,22 @@ __device__ __forceinline__ double wave_bcast(double v, int lane) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
+/* Workgroups are dispatched to the 8 XCDs round-robin; this renumbering
+ * gives each XCD one contiguous range of workgroups, so the units of one
+ * system (consecutive numbers) share that XCD's L2 copy of its factor.
+ * A bijection of [0, gridDim.x). */
+__device__ __forceinline__ int xcd_block() {
+  const int n = (int)gridDim.x, i = (int)blockIdx.x;
+  const int q = n / 8, r = n % 8, x = i % 8, p = i / 8;
+  return (x < r ? x * (q + 1) : r * (q + 1) + (x - r) * q) + p;
+}
+
 template <int R, int CB, int D>
 __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __restrict__ d,
                                                             int count, int nunits) {
   static_assert(64 % D == 0, "step blocks stay inside a row group");
   const int lane = threadIdx.x & 63;
-  const int unit = (int)blockIdx.x * (TPB / 64) + (int)(threadIdx.x >> 6);
+  const int unit = xcd_block() * (TPB / 64) + (int)(threadIdx.x >> 6);
   if (unit >= nunits) return;  // (whole waves; nothing below synchronizes waves)
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -1341,6 +1351,211 @@ constexpr int wave_cols(int R) { return R <= 4 ? 8 : 4; }
 
 constexpr int MAX_GRID_Y = 65535;
 
+/* LU solves on the matrix cores (the factorization's leaf steps: systems of
+ * at most 16 RT rows, hundreds of columns).  One wave per system and 16
+ * columns of B, four independent waves per workgroup.  The wave keeps
+ * X = P B as RT tiles of 16 x 16 in the result layout of
+ * v_mfma_f64_16x16x4f64 (lane l, register i: row (l >> 4) + 4 i of the
+ * tile, column l & 15), which is also that instruction's B-operand layout
+ * for the tile's rows 4 i .. 4 i + 3: a solved tile multiplies the factor's
+ * off-diagonal blocks as it stands.  Forward (unit L) and backward (U) by
+ * tiles: the diagonal tile by substitution, row j's values sent to the lanes
+ * of their column by ds_bpermute and the factor's diagonal block staged in
+ * the wave's LDS; then every tile below (above) gets X_u -= F_ut X_t as four
+ * matrix-core products with the factor block read from global memory in the
+ * A-operand layout (lane l: row l & 15, column l >> 4 of a 16 x 4 slice).
+ * The row interchanges come as one permutation (SolveDesc::perm: row i of
+ * P B is row perm[i] of B). */
+typedef double mf4d __attribute__((ext_vector_type(4)));
+
+template <int RT>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))) lu_solve_mfma_kernel(const SolveDesc* __restrict__ d,
+                                                            int count, int nunits) {
+  __shared__ double Ds[TPB / 64][16][17];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  const int unit = xcd_block() * (TPB / 64) + w;
+  if (unit >= nunits) return;  // (whole waves; nothing below synchronizes waves)
+  int lo = 0, hi = count - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (d[mid].chunk0 <= unit)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const SolveDesc p = d[lo];
+  const int n = p.n, nt = (n + 15) >> 4;
+  const int q = lane >> 4, c = lane & 15;
+  const size_t ld = (size_t)p.ld_lu;
+  const double* __restrict__ F = p.LU;
+  double(*D)[17] = Ds[w];
+  // P B tile by tile through the wave's LDS block: lane l reads row l & 15 of
+  // the tile in columns 4 (l >> 4) + i (16 consecutive rows of a column per
+  // quarter wave), and takes back its result-layout values
+  const int c0 = (unit - p.chunk0) * 16;
+  const int lr = lane & 15, lc = (lane >> 4) * 4;
+  mf4d x[RT];
+#pragma unroll
+  for (int t = 0; t < RT; t++) {
+    const int r = 16 * t + lr;
+    const int src = r < n ? p.perm[r] : 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int cc = c0 + lc + i;
+      D[lr][lc + i] = (r < n && cc < p.ncols) ? p.B[(size_t)src + (size_t)cc * p.ldb] : 0.0;
+    }
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < 4; i++) x[t][i] = D[q + 4 * i][c];
+    __builtin_amdgcn_wave_barrier();
+  }
+  // Forward, tile T = 0 .. RT-1: the tile being solved is always x[0] and
+  // x[u] holds tile T + u; after each step the tiles rotate down by one (the
+  // solved tile to x[RT-1]), so after RT steps every tile is back in place.
+  // (Fixed register positions keep the loop body small: fully unrolled over
+  // the tiles the code outgrows the instruction cache.)
+  for (int T = 0; T < RT; T++) {
+    if (T < nt) {
+#pragma unroll
+      for (int e4 = 0; e4 < 4; e4++) {
+        const int e = lane + 64 * e4, rr = e & 15, cc = e >> 4;
+        const int gr = 16 * T + rr, gc = 16 * T + cc;
+        D[rr][cc] = (gr < n && gc < n) ? F[(size_t)gr + (size_t)gc * ld] : 0.0;
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = 0; j < 15; j++) {
+        const double xj = __shfl(x[0][j >> 2], ((j & 3) << 4) | c, 64);
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int r = q + 4 * i;
+          if (r > j) x[0][i] -= D[r][j] * xj;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      // the factor block of tile u is loaded one tile ahead of its products
+      const double* Fc = F + (size_t)(16 * T + q) * ld;  // column 16 T + q
+      auto fwd_load = [&](int u, double (&a)[4]) {
+        const int gr = 16 * (T + u) + c;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+          const int gc = 16 * T + 4 * kk + q;
+          a[kk] = (T + u < nt && gr < n && gc < n) ? -Fc[(size_t)gr + (size_t)(4 * kk) * ld] : 0.0;
+        }
+      };
+      double a[4], an[4];
+      fwd_load(1, a);
+#pragma unroll
+      for (int u = 1; u < RT; u++) {
+        if (u + 1 < RT) fwd_load(u + 1, an);
+        if (T + u < nt) {
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++)
+            x[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], x[0][kk], x[u], 0, 0, 0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) a[kk] = an[kk];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    const mf4d s0 = x[0];
+#pragma unroll
+    for (int u = 0; u < RT - 1; u++) x[u] = x[u + 1];
+    x[RT - 1] = s0;
+  }
+  // Backward, tile T = RT-1 .. 0: the tile being solved is always x[RT-1]
+  // and x[RT-1-v] holds tile T - v; the tiles rotate up by one per step.
+#pragma unroll 1
+  for (int T = RT - 1; T >= 0; T--) {
+    if (T < nt) {
+#pragma unroll
+      for (int e4 = 0; e4 < 4; e4++) {
+        const int e = lane + 64 * e4, rr = e & 15, cc = e >> 4;
+        const int gr = 16 * T + rr, gc = 16 * T + cc;
+        D[rr][cc] = (gr < n && gc < n) ? F[(size_t)gr + (size_t)gc * ld] : (gr == gc ? 1.0 : 0.0);
+      }
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int j = 15; j >= 0; j--) {
+        const double xj = __shfl(x[RT - 1][j >> 2], ((j & 3) << 4) | c, 64) / D[j][j];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int r = q + 4 * i;
+          if (r == j)
+            x[RT - 1][i] = xj;
+          else if (r < j)
+            x[RT - 1][i] -= D[r][j] * xj;
+        }
+      }
+      __builtin_amdgcn_wave_barrier();
+      const double* Fc = F + (size_t)(16 * T + q) * ld;
+      auto bwd_load = [&](int v, double (&a)[4]) {
+        const int gr = 16 * (T - v) + c;
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) {
+          const int gc = 16 * T + 4 * kk + q;
+          a[kk] = (v <= T && gc < n) ? -Fc[(size_t)gr + (size_t)(4 * kk) * ld] : 0.0;
+        }
+      };
+      double a[4], an[4];
+      bwd_load(1, a);
+#pragma unroll
+      for (int v = 1; v < RT; v++) {
+        if (v + 1 < RT) bwd_load(v + 1, an);
+        if (v <= T) {
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++)
+            x[RT - 1 - v] =
+                __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], x[RT - 1][kk], x[RT - 1 - v], 0, 0, 0);
+        }
+#pragma unroll
+        for (int kk = 0; kk < 4; kk++) a[kk] = an[kk];
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    const mf4d s1 = x[RT - 1];
+#pragma unroll
+    for (int u = RT - 1; u > 0; u--) x[u] = x[u - 1];
+    x[0] = s1;
+  }
+  // back through LDS: quarter waves write 16 consecutive rows of a column
+#pragma unroll
+  for (int t = 0; t < RT; t++) {
+    __builtin_amdgcn_wave_barrier();
+#pragma unroll
+    for (int i = 0; i < 4; i++) D[q + 4 * i][c] = x[t][i];
+    __builtin_amdgcn_wave_barrier();
+    const int r = 16 * t + lr;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int cc = c0 + lc + i;
+      if (r < n && cc < p.ncols) p.B[(size_t)r + (size_t)cc * p.ldb] = D[lr][lc + i];
+    }
+  }
+}
+
+/* The interchanges piv[0..n) of each matrix, applied in order to the
+ * identity, as one permutation (perm[i]: the row that ends in row i); one
+ * wave per matrix, the swaps done in order by lane 0 in LDS. */
+__global__ void __launch_bounds__(64) lu_perm_kernel(const PermBuild* __restrict__ d) {
+  __shared__ int pm[LU_PERM_MAX];
+  const PermBuild b = d[blockIdx.x];
+  const int lane = threadIdx.x;
+  for (int i = lane; i < b.n; i += 64) pm[i] = i;
+  __syncthreads();
+  if (lane == 0)
+    for (int k = 0; k < b.n; k++) {
+      const int q = b.piv[k];
+      if (q != k && (unsigned)q < (unsigned)b.n) {
+        const int t = pm[k];
+        pm[k] = pm[q];
+        pm[q] = t;
+      }
+    }
+  __syncthreads();
+  for (int i = lane; i < b.n; i += 64) b.perm[i] = pm[i];
+}
+
 }  // namespace
 
 void launch_assemble(const AssembleDesc* d, int count, int max_elems, const EntrySource* srcs,
@@ -1393,6 +1608,28 @@ void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStrea
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
   trsm_kernel<<<nwg, 64, 0, s>>>(d, count);
+  HODLR_HIP_CHECK(hipGetLastError());
+}
+
+int lu_solve_mfma_units(int ncols) { return (ncols + 15) / 16; }
+
+bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s) {
+  if (maxn > LU_SOLVE_MFMA_MAX) return false;
+  if (count == 0 || nunits == 0) return true;
+  const int grid = (nunits + TPB / 64 - 1) / (TPB / 64);
+  if (maxn <= 64)
+    lu_solve_mfma_kernel<4><<<grid, TPB, 0, s>>>(d, count, nunits);
+  else if (maxn <= 128)
+    lu_solve_mfma_kernel<8><<<grid, TPB, 0, s>>>(d, count, nunits);
+  else
+    lu_solve_mfma_kernel<16><<<grid, TPB, 0, s>>>(d, count, nunits);
+  HODLR_HIP_CHECK(hipGetLastError());
+  return true;
+}
+
+void launch_lu_perm(const PermBuild* d, int count, hipStream_t s) {
+  if (count == 0) return;
+  lu_perm_kernel<<<count, 64, 0, s>>>(d);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

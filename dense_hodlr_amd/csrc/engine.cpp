@@ -83,8 +83,8 @@ void Engine::set_tuning(const Tuning& t) {
   if (t.gemm_target < 1) throw std::invalid_argument("split-K target must be positive");
   if (t.lu_nb < 1 || t.lu_nb > LU_NB) throw std::invalid_argument("LU panel width must be 1..64");
   if (t.gemm_mfma != 0 && t.gemm_mfma != 1) throw std::invalid_argument("mfma must be 0 or 1");
-  if (t.leaf_solve < 0 || t.leaf_solve > 2)
-    throw std::invalid_argument("the leaf solve variant must be 0, 1 or 2");
+  if (t.leaf_solve < 0 || t.leaf_solve > 3)
+    throw std::invalid_argument("the leaf solve variant must be 0, 1, 2 or 3");
   tune_ = t;
   tune_.tuned = false;
   reset_attributes();
@@ -136,9 +136,9 @@ void Engine::autotune() {
         fb = t;
       }
     }
-  {  // the leaf solves of the factorization: shared factor, blocked, one wave
+  {  // the leaf solves of the factorization: one wave, blocked, shared factor, matrix cores
     const Tuning base = fb;
-    for (int ls : {0, 1, 2}) {
+    for (int ls : {0, 1, 2, 3}) {
       if (ls == base.leaf_solve) continue;
       Tuning t = base;
       t.leaf_solve = ls;
@@ -452,7 +452,7 @@ long long Engine::device_bytes() const {
   long long b = 0;
   for (auto& x : U_) b += x.bytes();
   for (auto& x : V_) b += x.bytes();
-  b += Lorig_.bytes() + Lfac_.bytes() + Lpiv_.bytes();
+  b += Lorig_.bytes() + Lfac_.bytes() + Lpiv_.bytes() + Lperm_.bytes();
   for (auto& f : fac_) b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
   b += ws_.bytes() + tmp_.bytes() + tmp2_.bytes() + xw_.bytes();
   for (auto& f : mfac_) {
@@ -687,7 +687,20 @@ void Engine::lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& aren
 void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, int mode) {
   if (descs.empty()) return;
   int maxn = 0;
-  for (auto& d : descs) maxn = std::max(maxn, d.n);
+  bool perms = true;
+  for (auto& d : descs) {
+    maxn = std::max(maxn, d.n);
+    perms = perms && d.perm != nullptr;
+  }
+  if (mode == 3 && maxn <= LU_SOLVE_MFMA_MAX && perms) {
+    int nu = 0;
+    for (auto& d : descs) {
+      d.chunk0 = nu;
+      nu += lu_solve_mfma_units(d.ncols);
+    }
+    launch_lu_solve_mfma(arena_.push(descs, stream_), (int)descs.size(), nu, maxn, stream_);
+    return;
+  }
   if (mode == 2 && maxn <= LU_SOLVE_WAVE_MAX && maxn <= solve_wave_max_) {
     int nwg = 0;
     for (auto& d : descs) {
@@ -1346,6 +1359,12 @@ void Engine::factor_leaves() {
   const int t = prof_begin(stream_);
   lu_batch(ld);
   prof_end(t, stream_, FAC_LU, fl, by);
+  // the interchanges as permutations, for the matrix-core leaf solves
+  Lperm_.alloc(std::max<size_t>(leaf_piv_elems_, 1));
+  std::vector<PermBuild> pb;
+  for (auto& lf : leaves_)
+    if (lf.s <= LU_PERM_MAX) pb.push_back({Lpiv_.p + lf.piv_off, Lperm_.p + lf.piv_off, lf.s, 0});
+  launch_lu_perm(arena_.push(pb, stream_), (int)pb.size(), stream_);
   leaves_factored_ = true;
 }
 
@@ -1363,6 +1382,7 @@ void Engine::leaf_solves(const std::vector<int>& leaf_ids, double* B, int ldb, i
     SolveDesc s{};
     s.LU = Lfac_.p + lf.mat_off;
     s.piv = Lpiv_.p + lf.piv_off;
+    s.perm = lf.s <= LU_PERM_MAX ? Lperm_.p + lf.piv_off : nullptr;
     s.B = compact ? B + o : B + lf.row0;
     s.n = lf.s;
     s.ld_lu = lf.s;

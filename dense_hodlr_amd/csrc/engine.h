@@ -52,7 +52,8 @@ struct Tuning {
   int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
   int leaf_solve = 0;      // leaf solves of >= 128 columns (the factorization's): 0 one
                            // wave per column chunk, 1 blocked (diagonal-block substitution +
-                           // GEMM updates), 2 the factor shared by a workgroup's waves
+                           // GEMM updates), 2 the factor shared by a workgroup's waves,
+                           // 3 on the matrix cores (leaves of at most 256 rows)
   bool tuned = false;
 };
 
@@ -444,6 +445,7 @@ class Engine {
   int ktot_ = 0;
   DBuf<double> Lorig_, Lfac_;
   DBuf<int> Lpiv_;
+  DBuf<int> Lperm_;  // the leaves' interchanges as permutations (leaves up to LU_PERM_MAX rows)
   DBuf<int> lu_swp_;  // the batched LU's recorded panel interchanges (LU_SWP_INTS per matrix)
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;
   Factorization fac_[2];

@@ -6,7 +6,7 @@ import sys
 import numpy as np
 import pytest
 
-from dense_hodlr_amd import HODLR_Matrix
+from dense_hodlr_amd import HODLR_Matrix, HodlrNotConverged
 from dense_hodlr_amd import inputs as I
 
 pytestmark = pytest.mark.gpu
@@ -393,6 +393,8 @@ def test_svd_method_gauss1d_4096(gauss1d):
          gemm_mfma=True, leaf_solve=1),
     dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=4096, lu_panel=32,
          gemm_mfma=True, leaf_solve=2),
+    dict(aca_rows_per_thread=4, gemm_tile_m=0, gemm_split_target=4096, lu_panel=32,
+         gemm_mfma=True, leaf_solve=3),
 ])
 def test_launch_shapes_match_oracle(oracle, shapes):
     """Every launch-shape variant (ACA rows per thread, vector or matrix-core
@@ -530,3 +532,68 @@ def test_shared_factor_leaf_solves_match_wave_kernel(leaf):
     assert np.sum(n0[:, 4:]) > 128  # the leaf steps have many columns
     assert np.array_equal(xs0, xs1) and np.array_equal(xl0, xl1)
     assert d0 == d1
+
+
+@pytest.mark.parametrize("n,nrhs", [(37, 130), (64, 200), (130, 128), (250, 200), (256, 333)])
+def test_mfma_leaf_solve_matches_lapack(n, nrhs):
+    """The matrix-core LU solve (leaf_solve 3; solves of 128 or more columns
+    with leaves of at most 256 rows) on one dense leaf: a random matrix
+    (interchanges at almost every column), row counts off and on the 16-row
+    tiles, column counts off and on the 16-column chunks, against LAPACK's
+    solve of the same system."""
+    _case(f"matrix-core leaf solve n={n} nrhs={nrhs}")
+    rng = np.random.default_rng(1000 + n)
+    A = rng.standard_normal((n, n))
+    B = rng.standard_normal((n, nrhs))
+    g = HODLR_Matrix(A, 256)
+    g.set_launch_shapes(leaf_solve=3)
+    X = g.recSM_Solve(B)
+    Xr = np.linalg.solve(A, B)
+    err = np.max(np.abs(X - Xr)) / np.max(np.abs(Xr))
+    print(f"max error {err:.2e}, cond {np.linalg.cond(A):.1e}", flush=True)
+    assert err < 1e-11
+
+
+@pytest.mark.parametrize("leaf", [40, 100, 200, 256])
+def test_mfma_leaf_solves_match_oracle(oracle, leaf):
+    """The factorization's leaf steps on the matrix cores (leaf_solve 3): the
+    off-diagonal tiles are summed four terms at a time, so the results are
+    not bit-identical to the substitution kernels.  The ranks (decided before
+    the factorization) are the oracle's; the unrefined recSM and recLU
+    solutions agree with the oracle's to the rounding-order difference times
+    the conditioning (1e-9) with a backward error no larger than the
+    oracle's (double-double residuals against A_H); the refined solutions
+    (both sides refined with double-double residuals, which removes the
+    rounding-order dependence) agree to 1e-10; the log-det to 1e-11.
+    Ragged trees of 6000 points with leaves of up to 40, 100, 200 and 256
+    rows."""
+    _case(f"matrix-core leaf solves vs oracle, leaf {leaf}")
+    n = 6000
+    pts = I.points_2d_kd(n, 43)
+    b = I.rhs_normal(n, 3, 44)
+    g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.06, 1e-3, sizeThreshold=leaf)
+    g.set_launch_shapes(leaf_solve=3)
+    g.set_LRTolerance(1e-10)
+    xs, xl, ld = g.recSM_Solve(b), g.recLU_Solve(b), g.logAbsDeterminant()
+    o = oracle.OracleHODLR(n, leaf=leaf, points=pts, family="gaussian", length_scale=0.06,
+                           diag=1e-3)
+    o.set_LRTolerance(1e-10)
+    xo = o.recSM_Solve(b)
+    bn = np.linalg.norm(b, axis=0)
+    rg = np.max(np.linalg.norm(g.residual(b, xs), axis=0) / bn)
+    ro = np.max(np.linalg.norm(o.residual_dd(b, xo), axis=0) / bn)
+    xor, _, _ = o.refined_solve(b, 1e-14, 3)
+    g.set_precision("fp64_refined")
+    g.set_refinement(1e-14, 4)
+    try:
+        xr = g.recSM_Solve(b)
+    except HodlrNotConverged as e:  # the best iterate near the residual floor
+        xr = e.x
+    print(f"recSM {rel(xs, xo):.2e}, recLU {rel(xl, xo):.2e}, residual {rg:.2e} (oracle {ro:.2e}), "
+          f"refined {rel(xr, xor):.2e}", flush=True)
+    assert np.array_equal(g.nodes(), o.nodes())
+    assert np.sum(g.nodes()[:, 4:]) > 128
+    assert rel(xs, xo) < 1e-9 and rel(xl, xo) < 1e-9
+    assert rg <= 4.0 * ro + 1e-15
+    assert rel(xr, xor) < 1e-10
+    assert abs(ld - o.logAbsDeterminant()) <= 1e-11 * abs(o.logAbsDeterminant())
