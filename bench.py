@@ -394,12 +394,14 @@ def main():
         "ranks_max_per_level": st["max_rank"],
         "aca_iterations": st["aca_iterations"],
         "predictor": {"blocks": st["pred_blocks"], "misses": st["pred_misses"],
-                      "mean_abs_log_rank_error": st["pred_log_err"]},
+                      "mean_abs_log_rank_error": st["pred_log_err"],
+                      "panel_growths": st["aca_panel_grows"],
+                      "training": "2000 seeded runs (rank_mlp.npz, train_rank_model.py)"},
         "step_walls_s": walls,
         "plain_fp64": {"label": "refined, no predictor",
                        "value": plain, "step_walls_s": pwalls, "compress_s": pst["compress_time"],
                        "factor_s": pst["factor_time"], "solve_and_refine_s": pst["solve_time"],
-                       "residual": pst["last_residual"]},
+                       "residual": pst["last_residual"], "panel_growths": pst["aca_panel_grows"]},
         "speedup_vs_plain": plain / value,
         "tuning": tuning,
     }
@@ -444,7 +446,7 @@ def main():
         # the predictor on a system it was not trained on: Matern-3/2, l = 0.1,
         # N = 262144, tolerance 1e-10; compression only, with and without it
         dpts = I.points_2d_kd(n, 4242)
-        times = {}
+        times, grows, pbytes = {}, {}, {}
         for learned in (False, True, False, True):
             from dense_hodlr_amd import HODLR_Matrix
             g = HODLR_Matrix.from_kernel(dpts, "matern32", 0.1, 1e-3, sizeThreshold=256)
@@ -455,6 +457,8 @@ def main():
             g.storeLRinTree()
             times.setdefault(learned, []).append(time.perf_counter() - t0)
             sd = g.stats()
+            grows[learned] = sd["aca_panel_grows"]
+            pbytes[learned] = sd["aca_panel_bytes"]
             g = None
         out["learned_deploy"] = {
             "workload": f"matern32 2D kd N={n} l=0.1 leaf=256 tol=1e-10 (compression)",
@@ -462,7 +466,9 @@ def main():
             "speedup": min(times[False]) / min(times[True]),
             "blocks": sd["pred_blocks"], "misses": sd["pred_misses"],
             "miss_rate": sd["pred_misses"] / max(1, sd["pred_blocks"]),
-            "mean_abs_log_rank_error": sd["pred_log_err"], "max_rank": max(sd["max_rank"])}
+            "mean_abs_log_rank_error": sd["pred_log_err"], "max_rank": max(sd["max_rank"]),
+            "panel_growths": {"plain": grows[False], "predictor": grows[True]},
+            "panel_bytes": {"plain": pbytes[False], "predictor": pbytes[True]}}
 
     if not a.no_roofline:
         # one extra compression with HIP events around every ACA row/column

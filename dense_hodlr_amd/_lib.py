@@ -100,6 +100,8 @@ class Stats(C.Structure):
         ("fac_bytes", C.c_double * 4),
         ("fac_bound_time", C.c_double * 4),
         ("fac_launches", C.c_int * 4),
+        ("aca_panel_grows", C.c_int),
+        ("aca_panel_bytes", C.c_double),
     ]
 
 

@@ -744,6 +744,8 @@ int hodlr_get_stats(hodlr_handle h, hodlr_stats* out) {
       out->fac_bound_time[c] = s.fac_bound[c];
       out->fac_launches[c] = s.fac_launches[c];
     }
+    out->aca_panel_grows = s.aca_panel_grows;
+    out->aca_panel_bytes = s.aca_panel_bytes;
   });
 }
 

@@ -819,6 +819,7 @@ void Engine::compress() {
 }
 
 void Engine::aca_compress(std::vector<int>& blk_rank) {
+  stats_.aca_panel_grows = 0;
   const int nn = (int)nodes_.size();
   const int nl = std::max(1, nlev_ - 1);
   if (nn == 0) return;
@@ -1019,7 +1020,10 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
     }
     if (paused) {
       for (int l = 0; l < nl; l++)
-        if (grow[l]) alloc_level(l, std::min(upper_max[l], std::max(cap[l] + 1, 2 * cap[l])));
+        if (grow[l]) {
+          alloc_level(l, std::min(upper_max[l], std::max(cap[l] + 1, 2 * cap[l])));
+          stats_.aca_panel_grows++;
+        }
       for (int b : active)
         if (hs[b].status == 2) {
           hs[b].status = 0;
@@ -1044,7 +1048,12 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
     if (hs[b].k >= up - 1) need[blk[b].level] = std::max(need[blk[b].level], up);
   }
   for (int l = 0; l < nl; l++)
-    if (need[l] > cap[l]) alloc_level(l, need[l]);
+    if (need[l] > cap[l]) {
+      alloc_level(l, need[l]);
+      stats_.aca_panel_grows++;
+    }
+  stats_.aca_panel_bytes = 0;
+  for (int l = 0; l < nl; l++) stats_.aca_panel_bytes += 2.0 * U_[l].bytes();
   for (int l = 0; l < nl; l++) P[l].release();
   aca_launch_post(L, d_blk.p, d_state.p, d_srcs_.p, d_ranks.p, nb, stream_);
   std::vector<int> ranks(nb);

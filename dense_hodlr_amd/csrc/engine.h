@@ -299,6 +299,10 @@ class Engine {
     int iter_iterations = 0;         // iterative solve: GMRES iterations,
     double iter_precond_residual = 0;  // final ||M^-1 (b - A x)|| / ||M^-1 b||,
     double iter_residual = 0;          // and ||b - A x|| / ||b|| (largest column)
+    // ACA panel growths (a level's U/V panels reallocated and copied when a
+    // block outgrew their capacity) and the panels' final bytes
+    int aca_panel_grows = 0;
+    double aca_panel_bytes = 0;
     int pred_blocks = 0, pred_misses = 0;  // with a rank predictor attached
     double pred_log_err = 0;               // mean |log predicted - log ACA rank|
     // with profiling on, the last FP64 factorization by kernel group

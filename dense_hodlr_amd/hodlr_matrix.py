@@ -319,6 +319,7 @@ class HODLR_Matrix:
                     graph_launches=s.graph_launches, iter_iterations=s.iter_iterations,
                     iter_precond_residual=s.iter_precond_residual,
                     iter_residual=s.iter_residual,
+                    aca_panel_grows=s.aca_panel_grows, aca_panel_bytes=s.aca_panel_bytes,
                     factor_groups={
                         name: dict(time=s.fac_time[i], flops=s.fac_flops[i],
                                    bytes=s.fac_bytes[i], bound_time=s.fac_bound_time[i],

@@ -152,6 +152,12 @@ typedef struct hodlr_stats {
   double fac_bytes[4];
   double fac_bound_time[4];
   int fac_launches[4];
+  /* the last ACA compression: panel growths (a level's U/V panels
+   * reallocated and copied because a block outgrew their capacity: the
+   * capacity starts at 32 columns, or at the rank predictor's sizes) and the
+   * bytes of the U/V panels it ended with */
+  int aca_panel_grows;
+  double aca_panel_bytes;
 } hodlr_stats;
 
 /* One node of a user-defined partition: user_IndexTree::node

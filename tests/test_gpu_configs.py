@@ -135,12 +135,16 @@ def test_predictor_deployed_on_matern_262144():
     """The predictor on a system outside its training set: Matern-3/2,
     l = 0.1, N = 262144, tolerance 1e-10.  ACA still decides every rank
     (identical ranks with and without it); the fraction of blocks whose
-    rank exceeds the predicted capacity stays small, and the predicted
-    sizing does not make compression slower."""
+    rank exceeds the predicted capacity stays small.  What it gains: the
+    plain ACA starts every level's panels at 32 columns and reallocates and
+    copies them each time a block outgrows them (to the next doubling); the
+    predictor-sized panels are allocated once at 1.25 x the predicted ranks,
+    so the growths are gone, the panels end smaller, and compression is not
+    slower."""
     _case("matern32 262144 predictor deployment")
     pts = I.points_2d_kd(262144, 4242)
     model = RankModel.load()
-    times, nodes, st = {}, {}, None
+    times, nodes, st, grows, pbytes = {}, {}, None, {}, {}
     for learned in (False, True, False, True):
         g = HODLR_Matrix.from_kernel(pts, "matern32", 0.1, 1e-3, sizeThreshold=256)
         g.set_LRTolerance(1e-10)
@@ -150,6 +154,8 @@ def test_predictor_deployed_on_matern_262144():
         s = g.stats()
         times.setdefault(learned, []).append(s["compress_time"])
         nodes[learned] = g.nodes()
+        grows[learned] = s["aca_panel_grows"]
+        pbytes[learned] = s["aca_panel_bytes"]
         if learned:
             st = s
         del g
@@ -157,7 +163,10 @@ def test_predictor_deployed_on_matern_262144():
     miss = st["pred_misses"] / st["pred_blocks"]
     print(f"compression plain {plain:.4f} s, predictor-sized {pred:.4f} s "
           f"({plain / pred:.3f}x); misses {st['pred_misses']}/{st['pred_blocks']}, mean |log err| "
-          f"{st['pred_log_err']:.3f}", flush=True)
+          f"{st['pred_log_err']:.3f}; panel growths {grows[False]} -> {grows[True]}, panel bytes "
+          f"{pbytes[False] / 1e9:.2f} -> {pbytes[True] / 1e9:.2f} GB", flush=True)
     assert np.array_equal(nodes[False], nodes[True])
     assert miss <= 0.02
+    assert grows[False] >= 5 and grows[True] <= grows[False] // 5
+    assert pbytes[True] < pbytes[False]
     assert pred <= 1.01 * plain
