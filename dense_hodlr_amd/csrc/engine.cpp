@@ -62,6 +62,7 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipEventCreate(&ev_froot_));
   // (off: the roots' LU on the engine's stream, as every other level's)
   overlap_root_ = std::getenv("HODLR_NO_ROOT_OVERLAP") == nullptr;
+  adaptive_aca_pt_ = std::getenv("HODLR_FIXED_ACA_PT") == nullptr;
   build_trees();
   upload_sources();
   noalloc_setup();  // the allocation-free product's scratch address on this device
@@ -823,8 +824,9 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
   const int nn = (int)nodes_.size();
   const int nl = std::max(1, nlev_ - 1);
   if (nn == 0) return;
-  const int jpt = tune_.aca_pt, ipt = tune_.aca_pt;
-  const int cpw = 256 * jpt, rpw = 256 * ipt;
+  const int base_pt = tune_.aca_pt;
+  const int cpw = 256 * base_pt, rpw = 256 * base_pt;
+  int cur_pt = base_pt;  // panel rows per thread of the current launches
   const int nb = 2 * nn;
   std::vector<AcaBlock> blk(nb);
   std::vector<int> lvl_count(nl, 0), lvl_slots(nl, 0), upper_max(nl, 1);
@@ -932,7 +934,37 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
     if (blk[b].skip == 0 && blk[b].m > 0 && blk[b].n > 0) active.push_back(b);
   std::vector<int> m_row, m_col, m_fin, m_chunk, nchunk(nb, 1);
   AcaLaunch a{};
+  // When the active blocks would give fewer row-pass workgroups than the
+  // device has CUs (the last one or two blocks of the top level), their
+  // panel rows are spread over more workgroups (fewer rows per thread):
+  // the partial slots of the active blocks are laid out again, compactly per
+  // level (they only carry values within an iteration).
+  auto relayout = [&](int pt) {
+    std::vector<int> used(nl, 0);
+    for (int b : active) {
+      AcaBlock& x = blk[b];
+      x.row_nwg = (x.n + 256 * pt - 1) / (256 * pt);
+      x.col_nwg = (x.m + 256 * pt - 1) / (256 * pt);
+      x.part_off = used[x.level];
+      used[x.level] += x.row_nwg + x.col_nwg;
+    }
+    for (int l = 0; l < nl; l++)
+      if (used[l] > lvl_slots[l]) {
+        HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+        lvl_slots[l] = used[l];
+        P[l].alloc((size_t)lvl_slots[l] * (cap[l] + 4));
+        L.P[l] = P[l].p;
+      }
+    cur_pt = pt;
+  };
   auto build_maps = [&]() {
+    if (adaptive_aca_pt_) {
+      long long rw = 0;
+      for (int b : active) rw += (blk[b].n + 256 * base_pt - 1) / (256 * base_pt);
+      int pt = base_pt;
+      while (pt > 1 && rw > 0 && rw * (base_pt / pt) < aca_min_wg_) pt /= 2;
+      if (pt != cur_pt) relayout(pt);
+    }
     m_row.clear();
     m_col.clear();
     m_fin.clear();
@@ -967,8 +999,8 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
     a.n_row_wg = (int)m_row.size();
     a.n_col_wg = (int)m_col.size();
     a.n_fin_wg = (int)m_fin.size();
-    a.jpt = jpt;
-    a.ipt = ipt;
+    a.jpt = cur_pt;
+    a.ipt = cur_pt;
     a.tol = tol_;
     a.min_pivot = min_pivot_;
   };

@@ -450,6 +450,8 @@ class Engine {
   DBuf<double> Lorig_, Lfac_;
   DBuf<int> Lpiv_;
   DBuf<int> Lperm_;  // the leaves' interchanges as permutations (leaves up to LU_PERM_MAX rows)
+  bool adaptive_aca_pt_ = true;  // fewer ACA rows per thread when few workgroups remain
+  static constexpr int aca_min_wg_ = 256;  // ... fewer row-pass workgroups than this
   DBuf<int> lu_swp_;  // the batched LU's recorded panel interchanges (LU_SWP_INTS per matrix)
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;
   Factorization fac_[2];
