@@ -56,6 +56,7 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipEventCreate(&ev1_));
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_cap_, hipEventDisableTiming));
+  HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_aca_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_root_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreate(&ev_f0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev_f1_));
@@ -165,6 +166,7 @@ Engine::~Engine() {
   if (ev0_) (void)hipEventDestroy(ev0_);
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev_cap_) (void)hipEventDestroy(ev_cap_);
+  if (ev_aca_) (void)hipEventDestroy(ev_aca_);
   if (ev_root_) (void)hipEventDestroy(ev_root_);
   if (ev_f0_) (void)hipEventDestroy(ev_f0_);
   if (ev_f1_) (void)hipEventDestroy(ev_f1_);
@@ -1007,29 +1009,49 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
   build_maps();
 
   std::vector<AcaState> hs(nb);
+  // the state snapshot in pinned memory, so that its copy is asynchronous
+  struct PinnedStates {
+    AcaState* p = nullptr;
+    explicit PinnedStates(int n) {
+      HODLR_HIP_CHECK(hipHostMalloc((void**)&p, (size_t)std::max(n, 1) * sizeof(AcaState)));
+    }
+    ~PinnedStates() { (void)hipHostFree(p); }
+  } pin(nb);
   int check = 4;
   // every iteration either accepts a term or consumes an unchosen row, so a
   // block ends within m + n iterations
   long long max_dim = 0;
   for (auto& b : blk) max_dim = std::max<long long>(max_dim, (long long)b.m + b.n);
   long long launched = 0;
-  while (!active.empty()) {
-    if (launched > max_dim + 64)
-      throw std::runtime_error("ACA did not terminate within m + n iterations");
+  auto launch_batch = [&](int count) {
     if (profiling_) {
-      while ((int)prof_ev_.size() < 3 * check) {
+      while ((int)prof_ev_.size() < 3 * count) {
         hipEvent_t e;
         HODLR_HIP_CHECK(hipEventCreate(&e));
         prof_ev_.push_back(e);
       }
-      for (int c = 0; c < check; c++)
+      for (int c = 0; c < count; c++)
         aca_launch_iteration(L, a, d_srcs_.p, stream_, prof_ev_.data() + 3 * c);
     } else {
-      for (int c = 0; c < check; c++) aca_launch_iteration(L, a, d_srcs_.p, stream_);
+      for (int c = 0; c < count; c++) aca_launch_iteration(L, a, d_srcs_.p, stream_);
     }
-    launched += check;
-    HODLR_HIP_CHECK(hipMemcpyAsync(hs.data(), d_state.p, nb * sizeof(AcaState), hipMemcpyDeviceToHost, stream_));
-    HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
+    launched += count;
+  };
+  // Without profiling the next batch is launched before the state of the
+  // previous one is read (the GPU never waits for the host): blocks that
+  // finished or paused meanwhile skip it (their state says so), and the
+  // maps rebuilt from the state take effect with the batch after it.
+  const bool ahead = !profiling_;
+  launch_batch(check);
+  while (!active.empty()) {
+    if (launched > max_dim + 64)
+      throw std::runtime_error("ACA did not terminate within m + n iterations");
+    HODLR_HIP_CHECK(hipMemcpyAsync(pin.p, d_state.p, nb * sizeof(AcaState), hipMemcpyDeviceToHost, stream_));
+    HODLR_HIP_CHECK(hipEventRecord(ev_aca_, stream_));
+    const int next = std::min(16, check * 2);
+    if (ahead) launch_batch(next);
+    HODLR_HIP_CHECK(hipEventSynchronize(ev_aca_));
+    std::memcpy(hs.data(), pin.p, nb * sizeof(AcaState));
     if (profiling_) {
       for (int c = 0; c < check; c++) {
         float r = 0, q = 0;
@@ -1056,20 +1078,26 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
           alloc_level(l, std::min(upper_max[l], std::max(cap[l] + 1, 2 * cap[l])));
           stats_.aca_panel_grows++;
         }
+      // (a paused block made no progress since the snapshot: only its own
+      // state is written back)
       for (int b : active)
         if (hs[b].status == 2) {
           hs[b].status = 0;
+          pin.p[b].status = 0;
           na.push_back(b);
+          HODLR_HIP_CHECK(hipMemcpyAsync(d_state.p + b, pin.p + b, sizeof(AcaState),
+                                         hipMemcpyHostToDevice, stream_));
         }
       std::sort(na.begin(), na.end());
-      HODLR_HIP_CHECK(hipMemcpyAsync(d_state.p, hs.data(), nb * sizeof(AcaState), hipMemcpyHostToDevice, stream_));
     }
     if (na.empty()) break;
     const bool changed = paused || na.size() != active.size();
     active.swap(na);
     if (changed) build_maps();
-    check = std::min(16, check * 2);
+    check = next;
+    if (!ahead) launch_batch(check);
   }
+  HODLR_HIP_CHECK(hipStreamSynchronize(stream_));
   int maxit = 0;
   for (int b = 0; b < nb; b++) maxit = std::max(maxit, hs[b].iters);
   stats_.aca_iterations = maxit;

@@ -431,6 +431,7 @@ class Engine {
   // follows the factorization works through the lower levels on stream_.
   hipStream_t aux_ = nullptr;
   hipEvent_t ev_cap_ = nullptr, ev_root_ = nullptr;
+  hipEvent_t ev_aca_ = nullptr;  // the ACA state snapshot's copy
   // timing of the factorization: its start on stream_, its end on stream_
   // and the end of the roots' LU on aux_ (read by sync_stats)
   hipEvent_t ev_f0_ = nullptr, ev_f1_ = nullptr, ev_froot_ = nullptr;
