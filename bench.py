@@ -57,7 +57,7 @@ def parse():
     p.add_argument("--launch-shapes", default=None,
                    help="fix the launch shapes instead of autotuning: ACA rows per thread, "
                         "GEMM tile rows, split-K target, LU panel, MFMA, leaf-solve variant "
-                        "(e.g. 4,0,4096,32,1,0)")
+                        "(e.g. 4,0,4096,32,1,4)")
     p.add_argument("--cpu-n", type=int, nargs="+", default=[2048, 4096, 8192, 16384],
                    help="oracle sample sizes (at least 2; the largest is the reported sample)")
     p.add_argument("--no-iterative", action="store_true",
@@ -124,7 +124,7 @@ def make_handle(pts, target, precision, max_steps):
     from dense_hodlr_amd import HODLR_Matrix
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
     if SHAPES:
-        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 0)
+        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 4)
     g.set_LRTolerance(1e-10)
     g.set_precision(precision)
     g.set_refinement(target, max_steps)
@@ -525,7 +525,7 @@ def main():
         from dense_hodlr_amd import HODLR_Matrix, HodlrNotConverged, _lib
         g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
         if SHAPES:
-            g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 0)
+            g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 4)
         t0 = time.perf_counter()
         status = "converged"
         try:

@@ -194,6 +194,14 @@ int lu_solve_shared_wgs(int ncols, int maxn);
 constexpr int LU_SOLVE_MFMA_MAX = 256;
 int lu_solve_mfma_units(int ncols);
 bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s);
+/* The one-wave solves with 16 columns per wave (twice the columns per factor
+ * load; bit-identical to launch_lu_solve_small's one-wave kernel) for
+ * systems of at most LU_SOLVE_WIDE_MAX rows; lu_solve_wide_units(ncols) per
+ * problem numbered by chunk0.  Returns false, and launches nothing, when
+ * maxn exceeds the limit. */
+constexpr int LU_SOLVE_WIDE_MAX = 256;
+int lu_solve_wide_units(int ncols);
+bool launch_lu_solve_wide(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s);
 /* One wave per job; n <= LU_PERM_MAX. */
 constexpr int LU_PERM_MAX = 4096;
 void launch_lu_perm(const PermBuild* d, int count, hipStream_t s);

@@ -1657,6 +1657,21 @@ int lu_solve_small_wgs(int ncols, int maxn, bool wave) {
   return (ncols + cb - 1) / cb;
 }
 
+int lu_solve_wide_units(int ncols) { return (ncols + 15) / 16; }
+
+bool launch_lu_solve_wide(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s) {
+  if (maxn > LU_SOLVE_WIDE_MAX) return false;
+  if (count == 0 || nunits == 0) return true;
+  const int grid = (nunits + TPB / 64 - 1) / (TPB / 64);
+  switch (wave_rows(maxn)) {
+    case 1: lu_solve_wave_kernel<1, 16, 8><<<grid, TPB, 0, s>>>(d, count, nunits); break;
+    case 2: lu_solve_wave_kernel<2, 16, 8><<<grid, TPB, 0, s>>>(d, count, nunits); break;
+    default: lu_solve_wave_kernel<4, 16, 4><<<grid, TPB, 0, s>>>(d, count, nunits); break;
+  }
+  HODLR_HIP_CHECK(hipGetLastError());
+  return true;
+}
+
 bool launch_lu_solve_small(const SolveDesc* d, int count, int nwg, int maxn, bool wave,
                            bool blocked, hipStream_t s) {
   if (maxn > LU_SOLVE_SMALL_MAX) return false;

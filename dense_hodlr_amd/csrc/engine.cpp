@@ -85,8 +85,8 @@ void Engine::set_tuning(const Tuning& t) {
   if (t.gemm_target < 1) throw std::invalid_argument("split-K target must be positive");
   if (t.lu_nb < 1 || t.lu_nb > LU_NB) throw std::invalid_argument("LU panel width must be 1..64");
   if (t.gemm_mfma != 0 && t.gemm_mfma != 1) throw std::invalid_argument("mfma must be 0 or 1");
-  if (t.leaf_solve < 0 || t.leaf_solve > 3)
-    throw std::invalid_argument("the leaf solve variant must be 0, 1, 2 or 3");
+  if (t.leaf_solve < 0 || t.leaf_solve > 4)
+    throw std::invalid_argument("the leaf solve variant must be 0 .. 4");
   tune_ = t;
   tune_.tuned = false;
   reset_attributes();
@@ -140,7 +140,7 @@ void Engine::autotune() {
     }
   {  // the leaf solves of the factorization: one wave, blocked, shared factor, matrix cores
     const Tuning base = fb;
-    for (int ls : {0, 1, 2, 3}) {
+    for (int ls : {0, 1, 2, 3, 4}) {
       if (ls == base.leaf_solve) continue;
       Tuning t = base;
       t.leaf_solve = ls;
@@ -694,6 +694,15 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, int mode) {
   for (auto& d : descs) {
     maxn = std::max(maxn, d.n);
     perms = perms && d.perm != nullptr;
+  }
+  if (mode == 4 && maxn <= LU_SOLVE_WIDE_MAX && maxn <= solve_wave_max_) {
+    int nu = 0;
+    for (auto& d : descs) {
+      d.chunk0 = nu;
+      nu += lu_solve_wide_units(d.ncols);
+    }
+    launch_lu_solve_wide(arena_.push(descs, stream_), (int)descs.size(), nu, maxn, stream_);
+    return;
   }
   if (mode == 3 && maxn <= LU_SOLVE_MFMA_MAX && perms) {
     int nu = 0;

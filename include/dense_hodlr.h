@@ -130,8 +130,9 @@ typedef struct hodlr_stats {
   int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
   double factor_stream_time; /* the factorization up to its last kernel on the engine's
                                 stream, where a following solve starts */
-  int tune_leaf_solve;    /* the factorization's leaf solves: 0 one wave per column chunk,
-                             1 blocked, 2 factor shared per workgroup, 3 matrix cores */
+  int tune_leaf_solve;    /* the factorization's leaf solves: 0 one wave per 8 columns,
+                             1 blocked, 2 factor shared per workgroup, 3 matrix cores,
+                             4 one wave per 16 columns (the default) */
   /* the last iterative solve: GMRES iterations, the final preconditioned
    * relative residual ||M^-1 (b - A x)|| / ||M^-1 b|| it stopped on, and the
    * true one ||b - A x|| / ||b|| (largest column, FP64) */
@@ -286,8 +287,9 @@ int hodlr_autotune(hodlr_handle h);
  * launch, one wave per column chunk, 1 blocked (diagonal blocks by
  * substitution, the rest by GEMM updates), 2 one launch with the factor
  * shared by the waves of a workgroup, 3 one launch on the matrix cores
- * (leaves of at most 256 rows; others as 0).  Drops cached compression and
- * factorization.
+ * (leaves of at most 256 rows; others as 0), 4 (the default) as 0 with 16
+ * columns per wave (leaves of at most 256 rows; others as 0; bit-identical
+ * to 0).  Drops cached compression and factorization.
  * No reference counterpart. */
 int hodlr_set_launch_shapes(hodlr_handle h, int aca_rows_per_thread, int gemm_tile_m,
                             int gemm_split_target, int lu_panel, int gemm_mfma, int leaf_solve);

@@ -511,27 +511,29 @@ def test_matern2d_65536_against_oracle_summary():
 @pytest.mark.parametrize("leaf", [40, 100, 200, 400])
 def test_shared_factor_leaf_solves_match_wave_kernel(leaf):
     """The factorization's leaf steps (hundreds of columns) solve with one
-    wave per 8-column chunk streaming the whole factor (leaf_solve 0, the
-    default) or with the factor shared by a workgroup's four waves (2):
-    the same operations in the same order per element, so recSM, recLU and
-    the log-det are bit-identical.  Leaf sizes 40/100/200/400 (ragged trees
+    wave per 8-column chunk streaming the whole factor (leaf_solve 0), with
+    the factor shared by a workgroup's four waves (2), or with one wave per
+    16-column chunk (4, the default for leaves of at most 256 rows): the same
+    operations in the same order per element, so recSM, recLU and the log-det
+    are bit-identical.  Leaf sizes 40/100/200/400 (ragged trees
     of 6000 points) cover one, two, four and eight rows per lane."""
     _case(f"shared-factor leaf solves vs one wave, leaf {leaf}")
     n = 6000
     pts = I.points_2d_kd(n, 43)
     b = I.rhs_normal(n, 3, 44)
     out = []
-    for ls in (0, 2):
+    for ls in (0, 2, 4):
         g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.06, 1e-3, sizeThreshold=leaf)
         g.set_launch_shapes(leaf_solve=ls)
         g.set_LRTolerance(1e-10)
         out.append((g.recSM_Solve(b), g.recLU_Solve(b), g.logAbsDeterminant(), g.nodes()))
         del g
-    (xs0, xl0, d0, n0), (xs1, xl1, d1, n1) = out
-    assert np.array_equal(n0, n1)
+    (xs0, xl0, d0, n0) = out[0]
     assert np.sum(n0[:, 4:]) > 128  # the leaf steps have many columns
-    assert np.array_equal(xs0, xs1) and np.array_equal(xl0, xl1)
-    assert d0 == d1
+    for xs1, xl1, d1, n1 in out[1:]:
+        assert np.array_equal(n0, n1)
+        assert np.array_equal(xs0, xs1) and np.array_equal(xl0, xl1)
+        assert d0 == d1
 
 
 @pytest.mark.parametrize("n,nrhs", [(37, 130), (64, 200), (130, 128), (250, 200), (256, 333)])
