@@ -124,7 +124,7 @@ def make_handle(pts, target, precision, max_steps):
     from dense_hodlr_amd import HODLR_Matrix
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
     if SHAPES:
-        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 4)
+        g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 3)
     g.set_LRTolerance(1e-10)
     g.set_precision(precision)
     g.set_refinement(target, max_steps)
@@ -525,7 +525,7 @@ def main():
         from dense_hodlr_amd import HODLR_Matrix, HodlrNotConverged, _lib
         g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
         if SHAPES:
-            g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 4)
+            g.set_launch_shapes(*SHAPES[:4], bool(SHAPES[4]), SHAPES[5] if len(SHAPES) > 5 else 3)
         t0 = time.perf_counter()
         status = "converged"
         try:

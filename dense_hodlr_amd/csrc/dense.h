@@ -186,14 +186,14 @@ int lu_solve_small_wgs(int ncols, int maxn, bool wave);
  * and launches nothing, when maxn exceeds the limit. */
 int lu_solve_shared_wgs(int ncols, int maxn);
 /* The same solves on the matrix cores for systems of at most
- * LU_SOLVE_MFMA_MAX rows (SolveDesc::perm required): one wave per system and
- * 16 columns, lu_solve_mfma_units(ncols) per problem numbered by chunk0.
- * Not bit-identical to the substitution kernels (the off-diagonal tiles are
- * summed four terms at a time by the matrix cores).  Returns false, and
- * launches nothing, when maxn exceeds the limit. */
+ * LU_SOLVE_MFMA_MAX rows (SolveDesc::perm required): one workgroup per
+ * system and 64 columns, lu_solve_mfma_wgs(ncols) per problem numbered by
+ * chunk0.  Not bit-identical to the substitution kernels (the off-diagonal
+ * tiles are summed four terms at a time by the matrix cores).  Returns
+ * false, and launches nothing, when maxn exceeds the limit. */
 constexpr int LU_SOLVE_MFMA_MAX = 256;
-int lu_solve_mfma_units(int ncols);
-bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s);
+int lu_solve_mfma_wgs(int ncols);
+bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s);
 /* The one-wave solves with 16 columns per wave (twice the columns per factor
  * load; bit-identical to launch_lu_solve_small's one-wave kernel) for
  * systems of at most LU_SOLVE_WIDE_MAX rows; lu_solve_wide_units(ncols) per

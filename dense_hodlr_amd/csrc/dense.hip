@@ -1352,33 +1352,37 @@ constexpr int wave_cols(int R) { return R <= 4 ? 8 : 4; }
 constexpr int MAX_GRID_Y = 65535;
 
 /* LU solves on the matrix cores (the factorization's leaf steps: systems of
- * at most 16 RT rows, hundreds of columns).  One wave per system and 16
- * columns of B, four independent waves per workgroup.  The wave keeps
- * X = P B as RT tiles of 16 x 16 in the result layout of
- * v_mfma_f64_16x16x4f64 (lane l, register i: row (l >> 4) + 4 i of the
- * tile, column l & 15), which is also that instruction's B-operand layout
- * for the tile's rows 4 i .. 4 i + 3: a solved tile multiplies the factor's
- * off-diagonal blocks as it stands.  Forward (unit L) and backward (U) by
- * tiles: the diagonal tile by substitution, row j's values sent to the lanes
- * of their column by ds_bpermute and the factor's diagonal block staged in
- * the wave's LDS; then every tile below (above) gets X_u -= F_ut X_t as four
- * matrix-core products with the factor block read from global memory in the
- * A-operand layout (lane l: row l & 15, column l >> 4 of a 16 x 4 slice).
- * The row interchanges come as one permutation (SolveDesc::perm: row i of
- * P B is row perm[i] of B). */
+ * at most 16 RT rows, hundreds of columns).  A workgroup of four waves takes
+ * one system and 64 columns of B, 16 per wave.  A wave keeps X = P B as RT
+ * tiles of 16 x 16 in the result layout of v_mfma_f64_16x16x4f64 (lane l,
+ * register i: row (l >> 4) + 4 i of the tile, column l & 15), which is also
+ * that instruction's B-operand layout for the tile's rows 4 i .. 4 i + 3: a
+ * solved tile multiplies the factor's off-diagonal blocks as it stands.
+ * Forward (unit L) and backward (U) by tiles: the diagonal tile by
+ * substitution, row j's values sent to the lanes of their column by
+ * ds_bpermute; then every tile below (above) gets X_u -= F_ut X_t as four
+ * matrix-core products.  The factor's block column of step t (its diagonal
+ * tile and the tiles below it, or above it going backward) is staged in LDS
+ * by the whole workgroup in the A-operand layout (lane l: row l & 15, column
+ * l >> 4 of a 16 x 4 slice), and the next step's block column is loaded into
+ * registers while the current one is used: each factor element is read from
+ * memory once per 64 columns, off the substitution's critical path.  The row
+ * interchanges come as one permutation (SolveDesc::perm: row i of P B is row
+ * perm[i] of B).  Consecutive workgroups (the column chunks of one system)
+ * run on one XCD, whose L2 then holds the system's factor. */
 typedef double mf4d __attribute__((ext_vector_type(4)));
 
 template <int RT>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))) lu_solve_mfma_kernel(const SolveDesc* __restrict__ d,
-                                                            int count, int nunits) {
-  __shared__ double Ds[TPB / 64][16][17];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  const int unit = xcd_block() * (TPB / 64) + w;
-  if (unit >= nunits) return;  // (whole waves; nothing below synchronizes waves)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    lu_solve_mfma_kernel(const SolveDesc* __restrict__ d, int count) {
+  __shared__ double Pa[RT][4][64];        // block column: tile u, slice kk, A-operand lane
+  __shared__ double Ds[TPB / 64][16][17];  // each wave's transpose buffer for B
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+  const int blk = xcd_block();
   int lo = 0, hi = count - 1;
   while (lo < hi) {
     const int mid = (lo + hi + 1) >> 1;
-    if (d[mid].chunk0 <= unit)
+    if (d[mid].chunk0 <= blk)
       lo = mid;
     else
       hi = mid - 1;
@@ -1389,15 +1393,34 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
   const size_t ld = (size_t)p.ld_lu;
   const double* __restrict__ F = p.LU;
   double(*D)[17] = Ds[w];
+  // this thread's share of a block column: slice t >> 6, lane t & 63 of
+  // every tile: factor row 16 (t0 + u) + (t & 15), column 16 T + 4 (t >> 6)
+  // + ((t >> 4) & 3); rows and columns beyond n read as the identity
+  const int pr = t & 15, pc = 4 * (t >> 6) + ((t >> 4) & 3);
+  double pf[RT];
+  auto load_col = [&](int T, int t0, int cnt) {
+    const int gc = 16 * T + pc;
+#pragma unroll
+    for (int u = 0; u < RT; u++) {
+      const int gr = 16 * (t0 + u) + pr;
+      pf[u] = (u < cnt) ? ((gr < n && gc < n) ? F[(size_t)gr + (size_t)gc * ld] : (gr == gc ? 1.0 : 0.0))
+                        : 0.0;
+    }
+  };
+  auto store_col = [&](int cnt, bool reverse) {
+#pragma unroll
+    for (int u = 0; u < RT; u++)
+      if (u < cnt) Pa[reverse ? cnt - 1 - u : u][t >> 6][t & 63] = pf[u];
+  };
   // P B tile by tile through the wave's LDS block: lane l reads row l & 15 of
   // the tile in columns 4 (l >> 4) + i (16 consecutive rows of a column per
   // quarter wave), and takes back its result-layout values
-  const int c0 = (unit - p.chunk0) * 16;
+  const int c0 = (blk - p.chunk0) * 64 + 16 * w;
   const int lr = lane & 15, lc = (lane >> 4) * 4;
   mf4d x[RT];
 #pragma unroll
-  for (int t = 0; t < RT; t++) {
-    const int r = 16 * t + lr;
+  for (int tt = 0; tt < RT; tt++) {
+    const int r = 16 * tt + lr;
     const int src = r < n ? p.perm[r] : 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
@@ -1406,56 +1429,41 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int i = 0; i < 4; i++) x[t][i] = D[q + 4 * i][c];
+    for (int i = 0; i < 4; i++) x[tt][i] = D[q + 4 * i][c];
     __builtin_amdgcn_wave_barrier();
   }
   // Forward, tile T = 0 .. RT-1: the tile being solved is always x[0] and
   // x[u] holds tile T + u; after each step the tiles rotate down by one (the
   // solved tile to x[RT-1]), so after RT steps every tile is back in place.
   // (Fixed register positions keep the loop body small: fully unrolled over
-  // the tiles the code outgrows the instruction cache.)
+  // the tiles the code outgrows the instruction cache.)  Pa[u] holds the
+  // factor tile (T + u, T).
+  load_col(0, 0, nt);
   for (int T = 0; T < RT; T++) {
     if (T < nt) {
-#pragma unroll
-      for (int e4 = 0; e4 < 4; e4++) {
-        const int e = lane + 64 * e4, rr = e & 15, cc = e >> 4;
-        const int gr = 16 * T + rr, gc = 16 * T + cc;
-        D[rr][cc] = (gr < n && gc < n) ? F[(size_t)gr + (size_t)gc * ld] : 0.0;
-      }
-      __builtin_amdgcn_wave_barrier();
+      __syncthreads();  // every wave is done with the previous block column
+      store_col(nt - T, false);
+      __syncthreads();
+      if (T + 1 < nt) load_col(T + 1, T + 1, nt - T - 1);
 #pragma unroll
       for (int j = 0; j < 15; j++) {
         const double xj = __shfl(x[0][j >> 2], ((j & 3) << 4) | c, 64);
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int r = q + 4 * i;
-          if (r > j) x[0][i] -= D[r][j] * xj;
+          if (r > j) x[0][i] -= Pa[0][j >> 2][((j & 3) << 4) | r] * xj;
         }
       }
-      __builtin_amdgcn_wave_barrier();
-      // the factor block of tile u is loaded one tile ahead of its products
-      const double* Fc = F + (size_t)(16 * T + q) * ld;  // column 16 T + q
-      auto fwd_load = [&](int u, double (&a)[4]) {
-        const int gr = 16 * (T + u) + c;
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) {
-          const int gc = 16 * T + 4 * kk + q;
-          a[kk] = (T + u < nt && gr < n && gc < n) ? -Fc[(size_t)gr + (size_t)(4 * kk) * ld] : 0.0;
-        }
-      };
-      double a[4], an[4];
-      fwd_load(1, a);
 #pragma unroll
       for (int u = 1; u < RT; u++) {
-        if (u + 1 < RT) fwd_load(u + 1, an);
         if (T + u < nt) {
+          double a[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++) a[kk] = -Pa[u][kk][lane];
 #pragma unroll
           for (int kk = 0; kk < 4; kk++)
             x[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], x[0][kk], x[u], 0, 0, 0);
         }
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) a[kk] = an[kk];
-        __builtin_amdgcn_sched_barrier(0);
       }
     }
     const mf4d s0 = x[0];
@@ -1465,52 +1473,39 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
   // Backward, tile T = RT-1 .. 0: the tile being solved is always x[RT-1]
   // and x[RT-1-v] holds tile T - v; the tiles rotate up by one per step.
+  // Pa[v] holds the factor tile (T - v, T).
+  if (nt > 0) load_col(nt - 1, 0, nt);
 #pragma unroll 1
   for (int T = RT - 1; T >= 0; T--) {
     if (T < nt) {
-#pragma unroll
-      for (int e4 = 0; e4 < 4; e4++) {
-        const int e = lane + 64 * e4, rr = e & 15, cc = e >> 4;
-        const int gr = 16 * T + rr, gc = 16 * T + cc;
-        D[rr][cc] = (gr < n && gc < n) ? F[(size_t)gr + (size_t)gc * ld] : (gr == gc ? 1.0 : 0.0);
-      }
-      __builtin_amdgcn_wave_barrier();
+      __syncthreads();
+      store_col(T + 1, true);
+      __syncthreads();
+      if (T > 0) load_col(T - 1, 0, T);
 #pragma unroll
       for (int j = 15; j >= 0; j--) {
-        const double xj = __shfl(x[RT - 1][j >> 2], ((j & 3) << 4) | c, 64) / D[j][j];
+        const double xj = __shfl(x[RT - 1][j >> 2], ((j & 3) << 4) | c, 64) /
+                          Pa[0][j >> 2][((j & 3) << 4) | j];
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int r = q + 4 * i;
           if (r == j)
             x[RT - 1][i] = xj;
           else if (r < j)
-            x[RT - 1][i] -= D[r][j] * xj;
+            x[RT - 1][i] -= Pa[0][j >> 2][((j & 3) << 4) | r] * xj;
         }
       }
-      __builtin_amdgcn_wave_barrier();
-      const double* Fc = F + (size_t)(16 * T + q) * ld;
-      auto bwd_load = [&](int v, double (&a)[4]) {
-        const int gr = 16 * (T - v) + c;
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) {
-          const int gc = 16 * T + 4 * kk + q;
-          a[kk] = (v <= T && gc < n) ? -Fc[(size_t)gr + (size_t)(4 * kk) * ld] : 0.0;
-        }
-      };
-      double a[4], an[4];
-      bwd_load(1, a);
 #pragma unroll
       for (int v = 1; v < RT; v++) {
-        if (v + 1 < RT) bwd_load(v + 1, an);
         if (v <= T) {
+          double a[4];
+#pragma unroll
+          for (int kk = 0; kk < 4; kk++) a[kk] = -Pa[v][kk][lane];
 #pragma unroll
           for (int kk = 0; kk < 4; kk++)
             x[RT - 1 - v] =
                 __builtin_amdgcn_mfma_f64_16x16x4f64(a[kk], x[RT - 1][kk], x[RT - 1 - v], 0, 0, 0);
         }
-#pragma unroll
-        for (int kk = 0; kk < 4; kk++) a[kk] = an[kk];
-        __builtin_amdgcn_sched_barrier(0);
       }
     }
     const mf4d s1 = x[RT - 1];
@@ -1520,12 +1515,12 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
   }
   // back through LDS: quarter waves write 16 consecutive rows of a column
 #pragma unroll
-  for (int t = 0; t < RT; t++) {
+  for (int tt = 0; tt < RT; tt++) {
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
-    for (int i = 0; i < 4; i++) D[q + 4 * i][c] = x[t][i];
+    for (int i = 0; i < 4; i++) D[q + 4 * i][c] = x[tt][i];
     __builtin_amdgcn_wave_barrier();
-    const int r = 16 * t + lr;
+    const int r = 16 * tt + lr;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int cc = c0 + lc + i;
@@ -1611,18 +1606,17 @@ void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s) {
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
-int lu_solve_mfma_units(int ncols) { return (ncols + 15) / 16; }
+int lu_solve_mfma_wgs(int ncols) { return (ncols + 63) / 64; }
 
-bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nunits, int maxn, hipStream_t s) {
+bool launch_lu_solve_mfma(const SolveDesc* d, int count, int nwg, int maxn, hipStream_t s) {
   if (maxn > LU_SOLVE_MFMA_MAX) return false;
-  if (count == 0 || nunits == 0) return true;
-  const int grid = (nunits + TPB / 64 - 1) / (TPB / 64);
+  if (count == 0 || nwg == 0) return true;
   if (maxn <= 64)
-    lu_solve_mfma_kernel<4><<<grid, TPB, 0, s>>>(d, count, nunits);
+    lu_solve_mfma_kernel<4><<<nwg, TPB, 0, s>>>(d, count);
   else if (maxn <= 128)
-    lu_solve_mfma_kernel<8><<<grid, TPB, 0, s>>>(d, count, nunits);
+    lu_solve_mfma_kernel<8><<<nwg, TPB, 0, s>>>(d, count);
   else
-    lu_solve_mfma_kernel<16><<<grid, TPB, 0, s>>>(d, count, nunits);
+    lu_solve_mfma_kernel<16><<<nwg, TPB, 0, s>>>(d, count);
   HODLR_HIP_CHECK(hipGetLastError());
   return true;
 }

@@ -705,12 +705,12 @@ void Engine::lu_solve_batch(std::vector<SolveDesc>& descs, int mode) {
     return;
   }
   if (mode == 3 && maxn <= LU_SOLVE_MFMA_MAX && perms) {
-    int nu = 0;
+    int nwg = 0;
     for (auto& d : descs) {
-      d.chunk0 = nu;
-      nu += lu_solve_mfma_units(d.ncols);
+      d.chunk0 = nwg;
+      nwg += lu_solve_mfma_wgs(d.ncols);
     }
-    launch_lu_solve_mfma(arena_.push(descs, stream_), (int)descs.size(), nu, maxn, stream_);
+    launch_lu_solve_mfma(arena_.push(descs, stream_), (int)descs.size(), nwg, maxn, stream_);
     return;
   }
   if (mode == 2 && maxn <= LU_SOLVE_WAVE_MAX && maxn <= solve_wave_max_) {
@@ -1469,8 +1469,9 @@ void Engine::leaf_solves(const std::vector<int>& leaf_ids, double* B, int ldb, i
     sd.push_back(s);
     o += (size_t)lf.s * ncols;
   }
-  // many columns (the factorization's leaf step): the tuned variant
-  const int mode = ncols < 128 ? 0 : tune_.leaf_solve;
+  // many columns (the factorization's leaf step, solves of many right-hand
+  // sides): the tuned variant
+  const int mode = ncols < 64 ? 0 : tune_.leaf_solve;
   lu_solve_batch(sd, mode);
 }
 

@@ -50,11 +50,12 @@ struct Tuning {
   int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
   int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
   int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
-  int leaf_solve = 4;      // leaf solves of >= 128 columns (the factorization's): 0 one
-                           // wave per 8-column chunk, 1 blocked (diagonal-block substitution +
-                           // GEMM updates), 2 the factor shared by a workgroup's waves,
-                           // 3 on the matrix cores (leaves of at most 256 rows), 4 one wave
-                           // per 16-column chunk (leaves of at most 256 rows; else as 0)
+  int leaf_solve = 3;      // leaf solves of >= 64 columns (the factorization's and the solves
+                           // of many right-hand sides): 0 one wave per 8-column chunk,
+                           // 1 blocked (diagonal-block substitution + GEMM updates), 2 the
+                           // factor shared by a workgroup's waves, 3 on the matrix cores
+                           // (leaves of at most 256 rows; else as 0), 4 one wave per
+                           // 16-column chunk (leaves of at most 256 rows; else as 0)
   bool tuned = false;
 };
 

@@ -130,9 +130,9 @@ typedef struct hodlr_stats {
   int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
   double factor_stream_time; /* the factorization up to its last kernel on the engine's
                                 stream, where a following solve starts */
-  int tune_leaf_solve;    /* the factorization's leaf solves: 0 one wave per 8 columns,
-                             1 blocked, 2 factor shared per workgroup, 3 matrix cores,
-                             4 one wave per 16 columns (the default) */
+  int tune_leaf_solve;    /* leaf solves of 64 or more columns: 0 one wave per 8 columns,
+                             1 blocked, 2 factor shared per workgroup, 3 matrix cores
+                             (the default), 4 one wave per 16 columns */
   /* the last iterative solve: GMRES iterations, the final preconditioned
    * relative residual ||M^-1 (b - A x)|| / ||M^-1 b|| it stopped on, and the
    * true one ||b - A x|| / ||b|| (largest column, FP64) */
@@ -280,14 +280,15 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
 int hodlr_autotune(hodlr_handle h);
 
 /* Fixes the launch shapes instead of the autotuner's or the defaults (2, 0,
- * 2048, 64, 0, 0): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
+ * 2048, 64, 0, 3): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
  * (0: per launch, 64, 128), split-K target tiles, LU panel width (1..64),
  * FP64 GEMMs with 128-row tiles on the matrix cores (0, 1), and the
- * variant of the factorization's leaf solves (of 128 or more columns): 0 one
- * launch, one wave per column chunk, 1 blocked (diagonal blocks by
- * substitution, the rest by GEMM updates), 2 one launch with the factor
- * shared by the waves of a workgroup, 3 one launch on the matrix cores
- * (leaves of at most 256 rows; others as 0), 4 (the default) as 0 with 16
+ * variant of the leaf solves of 64 or more columns (the factorization's leaf
+ * steps, solves of many right-hand sides): 0 one launch, one wave per column
+ * chunk, 1 blocked (diagonal blocks by substitution, the rest by GEMM
+ * updates), 2 one launch with the factor shared by the waves of a workgroup,
+ * 3 (the default) one launch on the matrix cores, the factor shared by a
+ * workgroup (leaves of at most 256 rows; others as 0), 4 as 0 with 16
  * columns per wave (leaves of at most 256 rows; others as 0; bit-identical
  * to 0).  Drops cached compression and factorization.
  * No reference counterpart. */
