@@ -499,9 +499,10 @@ def main():
                          "every node of a level)",
                  "lu": "batched LU (lu_panel_reg_kernel + lu_step_kernel + trailing GEMM), "
                        "leaves and node matrices",
-                 "leaf_solve": "lu_solve_wave_kernel: leaf LU solves of the ancestors' U^SM "
-                               "columns",
-                 "node_solve": "node-matrix LU solves (lu_solve_wave/blk kernels)"}
+                 "leaf_solve": "leaf LU solves of the ancestors' U^SM columns (the tuned "
+                               "variant: lu_solve_mfma_kernel on the matrix cores by default)",
+                 "node_solve": "node-matrix LU solves (lu_solve_mfma_kernel up to 256 rows, "
+                               "lu_solve_wave/blk kernels above)"}
         more = []
         for key, label in names.items():
             q = fg[key]

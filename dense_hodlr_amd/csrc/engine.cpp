@@ -393,6 +393,7 @@ void Engine::reset_attributes() {
     f.W.release();
     f.C.release();
     f.piv.release();
+    f.perm.release();
   }
   for (auto& f : mfac_) f = MixedFactorization();
   U_.clear();
@@ -440,6 +441,7 @@ void Engine::set_precision(int p) {
       f.W.release();
       f.C.release();
       f.piv.release();
+      f.perm.release();
     }
     for (auto& f : mfac_) f = MixedFactorization();
   }
@@ -456,7 +458,7 @@ long long Engine::device_bytes() const {
   for (auto& x : U_) b += x.bytes();
   for (auto& x : V_) b += x.bytes();
   b += Lorig_.bytes() + Lfac_.bytes() + Lpiv_.bytes() + Lperm_.bytes();
-  for (auto& f : fac_) b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
+  for (auto& f : fac_) b += f.W.bytes() + f.C.bytes() + f.piv.bytes() + f.perm.bytes();
   b += ws_.bytes() + tmp_.bytes() + tmp2_.bytes() + xw_.bytes();
   for (auto& f : mfac_) {
     b += f.W.bytes() + f.C.bytes() + f.piv.bytes();
@@ -1521,6 +1523,8 @@ void Engine::factorize_fp64(int which) {
   }
   F.C.alloc(std::max<size_t>(co, 1));
   F.piv.alloc(std::max<size_t>(po, 1));
+  F.perm.alloc(std::max<size_t>(po, 1));
+  F.perm_level.assign(nlev_, 0);
   double* W = F.W.p;
   const int ld = ntot_;
   for (int d = nlev_ - 1; d >= 0; d--) {
@@ -1632,7 +1636,21 @@ void Engine::factorize_fp64(int which) {
         by += 8.0 * ((double)v.n * v.n + 2.0 * v.n * (double)v.ncols);
       }
       t = prof_begin(stream_);
-      lu_solve_batch(sd);
+      // node matrices of at most 256 rows: on the matrix cores with the leaf
+      // solves, their interchanges composed into permutations first
+      const bool mc = tune_.leaf_solve == 3 && maxK <= LU_SOLVE_MFMA_MAX;
+      if (mc) {
+        std::vector<PermBuild> pb;
+        for (size_t i = 0; i < sd.size(); i++) {
+          const int id = level_nodes_[d][i];
+          int* pm = F.perm.p + F.p_off[id];
+          pb.push_back({sd[i].piv, pm, sd[i].n, 0});
+          sd[i].perm = pm;
+        }
+        launch_lu_perm(arena_.push(pb, stream_), (int)pb.size(), stream_);
+        F.perm_level[d] = 1;
+      }
+      lu_solve_batch(sd, mc ? 3 : 0);
       prof_end(t, stream_, FAC_NODE_SOLVE, fl, by);
       gemm_work(gu, fl, by);
       t = prof_begin(stream_);
@@ -1815,7 +1833,14 @@ void Engine::solve_fp64_inplace(int which, double* X, int nrhs) {
     run_gemm(gz);
     launch_capacitance(arena_.push(cd, stream_), (int)cd.size(), stream_);
     if (offd == 0) wait_root(F);
-    lu_solve_batch(sd);
+    // many right-hand sides: the matrix cores where the factorization
+    // composed the level's permutations
+    const bool mc = nrhs >= 64 && (size_t)d < F.perm_level.size() && F.perm_level[d];
+    if (mc) {
+      const std::vector<int>& ids = level_nodes_[d];
+      for (size_t i = 0; i < sd.size(); i++) sd[i].perm = F.perm.p + F.p_off[ids[i]];
+    }
+    lu_solve_batch(sd, mc ? 3 : 0);
     run_gemm(gu);
   }
 }

@@ -226,6 +226,8 @@ struct Factorization {
   DBuf<double> W;
   DBuf<double> C;
   DBuf<int> piv;
+  DBuf<int> perm;  // node matrices' interchanges as permutations (matrix-core solves)
+  std::vector<char> perm_level;  // per level: perm holds its nodes' permutations
   std::vector<size_t> c_off;   // per node (global node order)
   std::vector<size_t> p_off;
 };
