@@ -495,7 +495,7 @@ def main():
         # vector units and on the matrix cores alike)
         g.recSM_Factorize()
         fg = g.stats()["factor_groups"]
-        names = {"gemm": "gemm_mfma_f64_kernel (grouped: V^T U^SM products and U^SM updates of "
+        names = {"gemm": "gemm_mfma_kernel<double> (grouped: V^T U^SM products and U^SM updates of "
                          "every node of a level)",
                  "lu": "batched LU (lu_panel_reg_kernel + lu_step_kernel + trailing GEMM), "
                        "leaves and node matrices",

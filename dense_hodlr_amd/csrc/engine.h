@@ -49,7 +49,7 @@ struct Tuning {
   int gemm_bm = 0;         // grouped GEMM tile rows (64, 128; 0: chosen per launch)
   int gemm_target = 2048;  // tiles a grouped GEMM aims for before splitting K
   int lu_nb = 64;          // panel width of the batched LU and triangular solves (<= 64)
-  int gemm_mfma = 0;       // FP64 grouped GEMM with 128-row tiles on the matrix cores
+  int gemm_mfma = 1;       // grouped GEMMs (FP64 and FP32) on the matrix cores
   int leaf_solve = 3;      // leaf solves of >= 64 columns (the factorization's and the solves
                            // of many right-hand sides): 0 one wave per 8-column chunk,
                            // 1 blocked (diagonal-block substitution + GEMM updates), 2 the

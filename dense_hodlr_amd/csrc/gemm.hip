@@ -172,32 +172,54 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   }
 }
 
-/* FP64 on the matrix cores: the descriptors and split-K contract of
- * gemm_kernel, tiles BM x BN of 128 x 64 or 64 x 128 (the latter for
- * products with few rows: the V^T W products of the factorization, whose
- * row count is a rank), each wave computing a 64 x 32 sub-tile as
- * 4 x 2 blocks of v_mfma_f64_16x16x4_f64.  Operand layout of that
- * instruction (wave of 64): lane l holds A[l % 16][l / 16] of the 16 x 4
+/* The matrix cores: the descriptors and split-K contract of gemm_kernel,
+ * tiles BM x BN of 128 x 64 or 64 x 128 (the latter for products with few
+ * rows: the V^T W products of the factorization, whose row count is a rank),
+ * each wave computing a 64 x 32 sub-tile as 4 x 2 blocks of
+ * v_mfma_f64_16x16x4_f64 (FP64) or v_mfma_f32_16x16x4_f32 (FP32, the mixed
+ * path's products, accumulated in FP32).  Operand layout of both
+ * instructions (wave of 64): lane l holds A[l % 16][l / 16] of the 16 x 4
  * A block and B[l / 16][l % 16] of the 4 x 16 B block; the 16 x 16 result
- * is held as D[l / 16 + 4 i][l % 16] in register i = 0..3.  The blocks are
- * multiplied transposed, D = B^T A^T = (A B)^T, so that lane l holds
- * (A B)[l % 16][l / 16 + 4 i]: the 16 lanes of a quarter wave then read and
- * write 16 consecutive rows of one column of the column-major C (a 128-byte
- * line) instead of one element in each of 16 columns. */
+ * is held in register i = 0..3 as D[l / 16 + 4 i][l % 16] (FP64) or
+ * D[4 (l / 16) + i][l % 16] (FP32).  The blocks are multiplied transposed,
+ * D = B^T A^T = (A B)^T, so that lane l holds (A B)[l % 16][Mfma<T>::col]:
+ * the 16 lanes of a quarter wave then read and write 16 consecutive rows of
+ * one column of the column-major C (a 64- or 128-byte run) instead of one
+ * element in each of 16 columns. */
 typedef double v4d __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
 
-template <int BM, int BN>
+template <class T>
+struct Mfma;
+template <>
+struct Mfma<double> {
+  typedef v4d acc;
+  static __device__ __forceinline__ acc mul(double a, double b, acc c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  // result column (of the 16) that register i of lane group g = l / 16 holds
+  static __device__ __forceinline__ int col(int g, int i) { return g + 4 * i; }
+};
+template <>
+struct Mfma<float> {
+  typedef v4f acc;
+  static __device__ __forceinline__ acc mul(float a, float b, acc c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int g, int i) { return 4 * g + i; }
+};
+
+template <class T, int BM, int BN>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-    gemm_mfma_f64_kernel(const GemmDescT<double>* __restrict__ descs, int count,
-                         double* __restrict__ ws) {
+    gemm_mfma_kernel(const GemmDescT<T>* __restrict__ descs, int count, T* __restrict__ ws) {
   constexpr int NT = 256, WM = BM / 64;  // waves along M
   static_assert(WM * (BN / 32) == 4, "four waves of 64 x 32");
   constexpr int A_PER = BM * BK / NT;
   constexpr int B_PER = BN * BK / NT;
-  __shared__ double As[BK][BM + 2];
-  __shared__ double Bs[BK][BN + 2];
+  __shared__ T As[BK][BM + 2];
+  __shared__ T Bs[BK][BN + 2];
   const int gtile = xcd_tile();
-  const GemmDescT<double> g = descs[find_problem(descs, count, gtile)];
+  const GemmDescT<T> g = descs[find_problem(descs, count, gtile)];
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   int tile = gtile - g.tile0;
   const int split = tile / (tm * tn);
@@ -212,21 +234,21 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   // C += A B and C -= A B (the factorization's updates) start the sums from
   // +-C, so the loads of C overlap the first operand slices instead of
   // following the last one; alpha = +-1 keeps every step exact but the sums
-  const bool cinit = g.ksplit == 1 && g.beta == 1.0 && (g.alpha == 1.0 || g.alpha == -1.0);
-  v4d acc[4][2];
+  const bool cinit = g.ksplit == 1 && g.beta == T(1) && (g.alpha == T(1) || g.alpha == T(-1));
+  typename Mfma<T>::acc acc[4][2];
 #pragma unroll
   for (int a = 0; a < 4; a++)
 #pragma unroll
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + lk + 4 * i;
+        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + Mfma<T>::col(lk, i);
         acc[a][b][i] = (cinit && gm < g.M && gn < g.N)
                            ? g.alpha * g.C[(size_t)gm + (size_t)gn * g.ldc]
-                           : 0.0;
+                           : T(0);
       }
 
-  double ra[A_PER], rb[B_PER];
+  T ra[A_PER], rb[B_PER];
   auto load = [&](int k0) {
 #pragma unroll
     for (int r = 0; r < A_PER; r++) {
@@ -242,14 +264,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       const int gm = m0 + mm, gk = k0 + kk;
       ra[r] = (gm < g.M && gk < kend)
                   ? (g.trans_a ? g.A[(size_t)gk + (size_t)gm * g.lda] : g.A[(size_t)gm + (size_t)gk * g.lda])
-                  : 0.0;
+                  : T(0);
     }
 #pragma unroll
     for (int r = 0; r < B_PER; r++) {
       const int e = t + r * NT;
       const int kk = e % BK, nn = e / BK;
       const int gn = n0 + nn, gk = k0 + kk;
-      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : 0.0;
+      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : T(0);
     }
   };
   auto store = [&]() {
@@ -284,7 +306,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     if (k0 + BK < kend) load(k0 + BK);
 #pragma unroll
     for (int kk = 0; kk < BK; kk += 4) {
-      double av[4], bv[2];
+      T av[4], bv[2];
 #pragma unroll
       for (int a = 0; a < 4; a++) av[a] = As[kk + lk][wm + a * 16 + lr];
 #pragma unroll
@@ -293,7 +315,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int b = 0; b < 2; b++)
-          acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a], acc[a][b], 0, 0, 0);
+          acc[a][b] = Mfma<T>::mul(bv[b], av[a], acc[a][b]);
     }
   }
 #pragma unroll
@@ -302,14 +324,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
     for (int b = 0; b < 2; b++)
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + lk + 4 * i;
+        const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + Mfma<T>::col(lk, i);
         if (gm < g.M && gn < g.N) {
-          const double v = acc[a][b][i];
+          const T v = acc[a][b][i];
           if (cinit) {
             g.C[(size_t)gm + (size_t)gn * g.ldc] = g.alpha * v;
           } else if (g.ksplit == 1) {
-            double* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
-            *c = (g.beta == 0.0) ? g.alpha * v : g.alpha * v + g.beta * *c;
+            T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+            *c = (g.beta == T(0)) ? g.alpha * v : g.alpha * v + g.beta * *c;
           } else {
             ws[g.ws_off + (size_t)split * g.M * g.N + (size_t)gm + (size_t)gn * g.M] = v;
           }
@@ -336,7 +358,7 @@ __global__ void __launch_bounds__(256)
 
 template <class T>
 int gemm_tile_m(const std::vector<GemmDescT<T>>& descs, bool mfma) {
-  if (std::is_same<T, double>::value && mfma) {
+  if (mfma) {
     // the same tile area either way: the shape that pads less
     double a128 = 0, a64 = 0;
     for (auto& g : descs) {
@@ -359,7 +381,7 @@ int gemm_tile_m(const std::vector<GemmDescT<T>>& descs, bool mfma) {
 
 template <class T>
 int gemm_tile_n(int bm, bool mfma) {
-  return (std::is_same<T, double>::value && mfma && bm == 64) ? 128 : 64;
+  return (mfma && bm == 64) ? 128 : 64;
 }
 
 template <class T>
@@ -402,20 +424,18 @@ template <class T>
 void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma) {
   if (count == 0 || ntiles == 0) return;
-  if constexpr (std::is_same<T, double>::value) {
-    if (mfma) {
-      if (bm == 128)
-        gemm_mfma_f64_kernel<128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
-      else
-        gemm_mfma_f64_kernel<64, 128><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+  if (mfma) {
+    if (bm == 128)
+      gemm_mfma_kernel<T, 128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+    else
+      gemm_mfma_kernel<T, 64, 128><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);
+    HODLR_HIP_CHECK(hipGetLastError());
+    if (nsplit > 0) {
+      dim3 g(64, nsplit);
+      splitk_reduce_kernel<T><<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
       HODLR_HIP_CHECK(hipGetLastError());
-      if (nsplit > 0) {
-        dim3 g(64, nsplit);
-        splitk_reduce_kernel<T><<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
-        HODLR_HIP_CHECK(hipGetLastError());
-      }
-      return;
     }
+    return;
   }
   if (bm == 128)
     gemm_kernel<T, 128, 64><<<ntiles, 256, 0, s>>>(d_descs, count, d_ws);

@@ -152,7 +152,7 @@ class HODLR_Matrix:
         check(lib().hodlr_autotune(self._h), self._h)
 
     def set_launch_shapes(self, aca_rows_per_thread=2, gemm_tile_m=0, gemm_split_target=2048,
-                          lu_panel=64, gemm_mfma=False, leaf_solve=3):
+                          lu_panel=64, gemm_mfma=True, leaf_solve=3):
         """Fix the launch shapes (see hodlr_set_launch_shapes)."""
         check(lib().hodlr_set_launch_shapes(self._h, int(aca_rows_per_thread), int(gemm_tile_m),
                                             int(gemm_split_target), int(lu_panel),

@@ -127,7 +127,7 @@ typedef struct hodlr_stats {
   int tune_gemm_split_target;
   int tune_lu_panel;
   int tuned;
-  int tune_gemm_mfma;     /* FP64 GEMMs on the matrix cores (MFMA) */
+  int tune_gemm_mfma;     /* grouped GEMMs on the matrix cores (MFMA) */
   double factor_stream_time; /* the factorization up to its last kernel on the engine's
                                 stream, where a following solve starts */
   int tune_leaf_solve;    /* leaf solves of 64 or more columns: 0 one wave per 8 columns,
@@ -280,9 +280,10 @@ int hodlr_set_refinement(hodlr_handle h, double target, int max_iters);
 int hodlr_autotune(hodlr_handle h);
 
 /* Fixes the launch shapes instead of the autotuner's or the defaults (2, 0,
- * 2048, 64, 0, 3): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
+ * 2048, 64, 1, 3): ACA panel rows per thread (1, 2, 4), grouped GEMM tile rows
  * (0: per launch, 64, 128), split-K target tiles, LU panel width (1..64),
- * FP64 GEMMs with 128-row tiles on the matrix cores (0, 1), and the
+ * grouped GEMMs (FP64, and the mixed path's FP32) on the matrix cores (1) or
+ * the vector units (0), and the
  * variant of the leaf solves of 64 or more columns (the factorization's leaf
  * steps, solves of many right-hand sides): 0 one launch, one wave per column
  * chunk, 1 blocked (diagonal blocks by substitution, the rest by GEMM

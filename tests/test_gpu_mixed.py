@@ -24,11 +24,16 @@ def colres(g, x, b):
     return np.max(np.linalg.norm(r, axis=0) / np.linalg.norm(b, axis=0))
 
 
+@pytest.mark.parametrize("mfma", [True, False])
 @pytest.mark.parametrize("which", ["SM", "LU"])
-def test_mixed_gauss1d_reaches_target(which):
-    _case(f"mixed gauss1d {which}")
+def test_mixed_gauss1d_reaches_target(which, mfma):
+    """The FP32 products of the mixed factorization and solve on the matrix
+    cores (v_mfma_f32_16x16x4_f32, FP32 accumulation; the default) or on the
+    vector units: either preconditioner brings the solution to 1e-12."""
+    _case(f"mixed gauss1d {which} mfma={mfma}")
     cfg = I.config_inputs("gauss1d_4096")
     g = HODLR_Matrix.from_kernel(cfg["points"], "gaussian", 0.1, 1e-2, sizeThreshold=64)
+    g.set_launch_shapes(gemm_mfma=mfma)
     g.set_LRTolerance(1e-10)
     solve = g.recSM_Solve if which == "SM" else g.recLU_Solve
     x64 = solve(cfg["rhs"])
@@ -42,16 +47,19 @@ def test_mixed_gauss1d_reaches_target(which):
     assert 1 <= st["refine_iterations"] <= 30
 
 
-@pytest.mark.parametrize("prec,budget", [("fp64_refined", 6), ("mixed", 40)])
-def test_refined_gauss2d_reaches_1e10(prec, budget):
+@pytest.mark.parametrize("prec,budget,mfma", [("fp64_refined", 6, True), ("mixed", 40, True),
+                                              ("mixed", 40, False)])
+def test_refined_gauss2d_reaches_1e10(prec, budget, mfma):
     """2D Gaussian, l=0.05, shift 1e-4 (condition number ~1e7): the plain FP64
     recSM solution stops near 1e-7; flexible GMRES refinement against A_H
-    brings it to 1e-10 with either preconditioner."""
-    _case(f"{prec} gauss2d 16384")
+    brings it to 1e-10 with either preconditioner, the mixed one's FP32
+    products on the matrix cores or on the vector units."""
+    _case(f"{prec} gauss2d 16384 mfma={mfma}")
     n = 16384
     pts = I.points_2d_kd(n, 5)
     b = I.rhs_normal(n, 8, 6)
     g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-4, sizeThreshold=256)
+    g.set_launch_shapes(gemm_mfma=mfma)
     g.set_LRTolerance(1e-10)
     x64 = g.recSM_Solve(b)
     r64 = colres(g, x64, b)

@@ -379,8 +379,10 @@ def test_svd_method_gauss1d_4096(gauss1d):
 
 
 @pytest.mark.parametrize("shapes", [
-    dict(aca_rows_per_thread=1, gemm_tile_m=64, gemm_split_target=1024, lu_panel=16),
-    dict(aca_rows_per_thread=4, gemm_tile_m=128, gemm_split_target=4096, lu_panel=32),
+    dict(aca_rows_per_thread=1, gemm_tile_m=64, gemm_split_target=1024, lu_panel=16,
+         gemm_mfma=False),
+    dict(aca_rows_per_thread=4, gemm_tile_m=128, gemm_split_target=4096, lu_panel=32,
+         gemm_mfma=False),
     dict(aca_rows_per_thread=2, gemm_tile_m=128, gemm_split_target=2048, lu_panel=64,
          gemm_mfma=True),
     dict(aca_rows_per_thread=2, gemm_tile_m=128, gemm_split_target=64, lu_panel=64,
@@ -412,7 +414,7 @@ def test_launch_shapes_match_oracle(oracle, shapes):
     assert np.array_equal(g.nodes(), o.nodes())
     assert rel(xs, xo) < 1e-10 and rel(xl, xo) < 1e-9
     assert abs(g.logAbsDeterminant() - o.logAbsDeterminant()) < 1e-11 * abs(o.logAbsDeterminant())
-    assert g.stats()["tuning"]["gemm_mfma"] == bool(shapes.get("gemm_mfma", False))
+    assert g.stats()["tuning"]["gemm_mfma"] == bool(shapes.get("gemm_mfma", True))
 
 
 @pytest.mark.parametrize("limit", [0, 40])
