@@ -471,6 +471,14 @@ def main():
             "panel_bytes": {"plain": pbytes[False], "predictor": pbytes[True]}}
 
     if not a.no_roofline:
+        # the sustained ceilings of this device (probe kernels), beside the
+        # datasheet peaks the fractions below are priced against
+        from dense_hodlr_amd import device_peaks
+        peaks = device_peaks()
+        out["device_peaks_measured"] = dict(
+            peaks, note="probe kernels on this device (hodlr_measure_device_peaks): streaming "
+                        "read of 1 GiB, 512 MiB copy, back-to-back v_mfma_f64_16x16x4, v_fma_f64 "
+                        "and v_mfma_f32_16x16x4 from registers; best of five launches")
         # one extra compression with HIP events around every ACA row/column
         # launch (the timed steps above run without them)
         g = make_handle(pts, a.target, "fp64_refined", 30)
@@ -489,6 +497,7 @@ def main():
             "avg_launch_us": {"row": 1e6 * sp["aca_row_time"] / iters,
                               "col": 1e6 * sp["aca_col_time"] / iters},
             "launches": iters,
+            "frac_of_measured_read": achieved / peaks["hbm_read_gbs"],
         }
         # the factorization of the same handle, its kernel groups timed with
         # HIP events on the streams they run on (FP64: 78.6 TF/s on the
@@ -514,7 +523,9 @@ def main():
                          "frac": tf / FP64_PEAK_TFS, "time_s": q["time"], "flops": q["flops"],
                          "bytes": q["bytes"], "launches": q["launches"],
                          "roofline_time_s": q["bound_time"],
-                         "frac_of_roofline_time": q["bound_time"] / q["time"]})
+                         "frac_of_roofline_time": q["bound_time"] / q["time"],
+                         "frac_of_measured_fp64": tf / (peaks["fp64_mfma_tflops"] if key == "gemm"
+                                                        else peaks["fp64_valu_tflops"])})
         out["roofline_factorization"] = more
         del g
 

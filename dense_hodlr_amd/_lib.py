@@ -42,7 +42,7 @@ EXPORTS = [
     "hodlr_rank_model_predict_bf16", "hodlr_rank_model_gradient", "hodlr_rank_model_train",
     "hodlr_block_features", "hodlr_set_rank_predictor",
     "hodlr_memory_get_stats", "hodlr_memory_trim", "hodlr_memory_cache",
-    "hodlr_read_binary_matrix", "hodlr_write_binary_matrix", "hodlr_read_text_matrix",
+    "hodlr_measure_device_peaks", "hodlr_read_binary_matrix", "hodlr_write_binary_matrix", "hodlr_read_text_matrix",
     "hodlr_create_dense_from_file", "hodlr_create_kernel_tree", "hodlr_create_dense_tree",
     "hodlr_kd_sort", "hodlr_kd_tree", "hodlr_diag_block", "hodlr_split_at_top",
     "hodlr_iterative_solve", "hodlr_iterative_solve_device", "hodlr_exact_apply_device",
@@ -103,6 +103,11 @@ class Stats(C.Structure):
         ("aca_panel_grows", C.c_int),
         ("aca_panel_bytes", C.c_double),
     ]
+
+
+class DevicePeaks(C.Structure):
+    _fields_ = [(name, C.c_double) for name in ("hbm_read_gbs", "hbm_copy_gbs", "fp64_mfma_tflops",
+                                                 "fp64_valu_tflops", "fp32_mfma_tflops")]
 
 
 class MemoryStats(C.Structure):
@@ -218,6 +223,7 @@ def lib():
         "hodlr_diag_block": (C.c_int, [h, C.c_int, hp]),
         "hodlr_split_at_top": (C.c_int, [h, hp, hp]),
         "hodlr_memory_get_stats": (C.c_int, [C.POINTER(MemoryStats)]),
+        "hodlr_measure_device_peaks": (C.c_int, [C.POINTER(DevicePeaks)]),
         "hodlr_memory_trim": (C.c_int, []),
         "hodlr_memory_cache": (C.c_int, [C.c_int]),
     }

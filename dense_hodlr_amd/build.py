@@ -28,7 +28,7 @@ COMMON = [
     "-Wall", "-Wno-unused-function", "-Wno-unused-result",
 ]
 
-SOURCES = ["aca.hip", "dense.hip", "gemm.hip", "ddres.hip", "svd.hip", "mlp.hip", "noalloc.hip", "matrix_io.cpp", "access.cpp", "kd.cpp", "exact.hip", "iterative.cpp", "pool.cpp", "engine.cpp", "model.cpp", "capi.cpp"]
+SOURCES = ["aca.hip", "dense.hip", "gemm.hip", "ddres.hip", "svd.hip", "mlp.hip", "noalloc.hip", "peaks.hip", "matrix_io.cpp", "access.cpp", "kd.cpp", "exact.hip", "iterative.cpp", "pool.cpp", "engine.cpp", "model.cpp", "capi.cpp"]
 
 
 def _compile(src: str) -> str:

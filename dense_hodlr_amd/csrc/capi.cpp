@@ -13,6 +13,7 @@
 #include "engine.h"
 #include "kd.h"
 #include "matrix_io.h"
+#include "peaks.h"
 #include "model.h"
 
 struct hodlr_handle_s {
@@ -799,6 +800,27 @@ const char* hodlr_last_error(hodlr_handle h) {
 }
 
 const char* hodlr_version(void) { return "dense_hodlr_amd 0.1 (gfx950)"; }
+
+int hodlr_measure_device_peaks(hodlr_device_peaks* out) {
+  if (!out) return fail(nullptr, HODLR_ERR_ARG, "null peaks pointer");
+  if (check_device() != 0) return fail(nullptr, HODLR_ERR_DEVICE, "no HIP device available");
+  try {
+    const hodlr::DevicePeaks p = hodlr::measure_device_peaks();
+    out->hbm_read_gbs = p.hbm_read_gbs;
+    out->hbm_copy_gbs = p.hbm_copy_gbs;
+    out->fp64_mfma_tflops = p.fp64_mfma_tflops;
+    out->fp64_valu_tflops = p.fp64_valu_tflops;
+    out->fp32_mfma_tflops = p.fp32_mfma_tflops;
+  } catch (const hodlr::HipError& e) {
+    const std::string m = e.what();
+    if (m.find("hipMalloc") != std::string::npos || m.find("out of memory") != std::string::npos)
+      return fail(nullptr, HODLR_ERR_NOMEM, m);
+    return fail(nullptr, HODLR_ERR_DEVICE, m);
+  } catch (const std::exception& e) {
+    return fail(nullptr, HODLR_ERR_DEVICE, e.what());
+  }
+  return HODLR_OK;
+}
 
 int hodlr_memory_get_stats(hodlr_memory_stats* out) {
   if (!out) return fail(nullptr, HODLR_ERR_ARG, "null stats pointer");

@@ -5,7 +5,7 @@ from __future__ import annotations
 
 import ctypes as C
 
-from ._lib import MemoryStats, check, lib
+from ._lib import DevicePeaks, MemoryStats, check, lib
 
 
 def memory_stats() -> dict:
@@ -24,3 +24,12 @@ def memory_trim() -> None:
 def memory_cache(on: bool) -> None:
     """Turn the cache off (emptying it) or back on."""
     check(lib().hodlr_memory_cache(1 if on else 0))
+
+
+def device_peaks() -> dict:
+    """Sustained throughput of the current device, measured by probe kernels
+    (hodlr_measure_device_peaks): HBM read and copy GB/s, FP64 matrix-core,
+    FP64 vector and FP32 matrix-core TFLOP/s."""
+    p = DevicePeaks()
+    check(lib().hodlr_measure_device_peaks(C.byref(p)))
+    return {name: getattr(p, name) for name, _ in DevicePeaks._fields_}

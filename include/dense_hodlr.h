@@ -500,6 +500,26 @@ const char* hodlr_last_error(hodlr_handle h);
 /* Library version string. */
 const char* hodlr_version(void);
 
+/* ---- device ceilings ---------------------------------------------------- */
+
+/* Sustained throughput of the current device, measured by probe kernels:
+ * a streaming read of 1 GiB (the access pattern of the ACA panel passes), a
+ * 512 MiB device copy (bytes read plus written), and back-to-back
+ * v_mfma_f64_16x16x4_f64, v_fma_f64 and v_mfma_f32_16x16x4_f32 from
+ * registers; best of five launches each.  These are the practical ceilings
+ * the engine's kernels are measured against beside the datasheet peaks (HBM
+ * 8 TB/s, FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s).  Takes about half a second
+ * and allocates 1 GiB for its duration.  No reference counterpart. */
+typedef struct hodlr_device_peaks {
+  double hbm_read_gbs;
+  double hbm_copy_gbs;
+  double fp64_mfma_tflops;
+  double fp64_valu_tflops;
+  double fp32_mfma_tflops;
+} hodlr_device_peaks;
+
+int hodlr_measure_device_peaks(hodlr_device_peaks* out);
+
 /* ---- device memory cache ----------------------------------------------- */
 
 /* Device blocks released by handles (destroyed, grown, refactored) are kept

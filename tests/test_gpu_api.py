@@ -176,3 +176,18 @@ def test_dense_constructor_from_binary_file(tmp_path, oracle):
     o = oracle.OracleHODLR(n, leaf=leaf, dense=A)
     o.set_LRTolerance(1e-10)
     assert rel(x2, o.recSM_Solve(b)) < 1e-10
+
+
+def test_device_peaks_are_sustained_ceilings():
+    """hodlr_measure_device_peaks: every probe returns a positive figure no
+    larger than the MI355X datasheet peak it is a sustained share of (HBM
+    8 TB/s, FP64 78.6 TFLOP/s, FP32 157.3 TFLOP/s), and the streaming read
+    reaches at least a quarter of the HBM peak."""
+    from dense_hodlr_amd import device_peaks
+    p = device_peaks()
+    print(p, flush=True)
+    assert 2000.0 < p["hbm_read_gbs"] <= 8000.0 * 1.02
+    assert 0.0 < p["hbm_copy_gbs"] <= 8000.0 * 1.02
+    assert 0.0 < p["fp64_mfma_tflops"] <= 78.6 * 1.02
+    assert 0.0 < p["fp64_valu_tflops"] <= 78.6 * 1.02
+    assert 0.0 < p["fp32_mfma_tflops"] <= 157.3 * 1.02
