@@ -64,6 +64,8 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   // (off: the roots' LU on the engine's stream, as every other level's)
   overlap_root_ = std::getenv("HODLR_NO_ROOT_OVERLAP") == nullptr;
   adaptive_aca_pt_ = std::getenv("HODLR_FIXED_ACA_PT") == nullptr;
+  // (off: the factorization's small-rank updates through the tiled GEMM)
+  stream_update_ = std::getenv("HODLR_NO_STREAM_UPDATE") == nullptr;
   build_trees();
   upload_sources();
   noalloc_setup();  // the allocation-free product's scratch address on this device
@@ -562,6 +564,33 @@ void Engine::prof_collect() {
 template <class T>
 void Engine::run_gemm_t(std::vector<GemmDescT<T>>& descs) {
   if (descs.empty()) return;
+  if constexpr (std::is_same<T, double>::value) {
+    // the low-rank updates of small rank: streamed on the matrix cores
+    // (bit-identical to the tiled kernel), the rest tiled
+    if (tune_.gemm_mfma && stream_update_) {
+      std::vector<GemmDesc> up, rest;
+      int maxk = 0;
+      for (auto& g : descs) {
+        if (gemm_update_eligible(g)) {
+          up.push_back(g);
+          maxk = std::max(maxk, g.K);
+        } else {
+          rest.push_back(g);
+        }
+      }
+      const int nwg = up.empty() ? 0 : gemm_update_plan(up, gemm_update_rows(maxk));
+      if (nwg >= 1024) {  // (fewer: the tiled kernel's shorter tiles fill the GPU better)
+        launch_gemm_update(arena_.push(up, stream_), (int)up.size(), nwg, maxk, stream_);
+        if (!rest.empty()) run_gemm_tiled(rest);
+        return;
+      }
+    }
+  }
+  run_gemm_tiled(descs);
+}
+
+template <class T>
+void Engine::run_gemm_tiled(std::vector<GemmDescT<T>>& descs) {
   std::vector<int> split_ids;
   int ntiles = 0;
   const bool mfma = tune_.gemm_mfma != 0;

@@ -371,6 +371,8 @@ class Engine {
   void run_gemm(std::vector<GemmDesc>& descs);
   template <class T>
   void run_gemm_t(std::vector<GemmDescT<T>>& descs);
+  template <class T>
+  void run_gemm_tiled(std::vector<GemmDescT<T>>& descs);
   void ensure_ws(size_t doubles);
   void grow(DBuf<double>& b, size_t doubles);
   void lu_batch(std::vector<LuDesc>& descs);
@@ -456,6 +458,7 @@ class Engine {
   DBuf<int> Lpiv_;
   DBuf<int> Lperm_;  // the leaves' interchanges as permutations (leaves up to LU_PERM_MAX rows)
   bool adaptive_aca_pt_ = true;  // fewer ACA rows per thread when few workgroups remain
+  bool stream_update_ = true;  // small-rank updates by the streaming kernel
   static constexpr int aca_min_wg_ = 256;  // ... fewer row-pass workgroups than this
   DBuf<int> lu_swp_;  // the batched LU's recorded panel interchanges (LU_SWP_INTS per matrix)
   size_t leaf_elems_ = 0, leaf_piv_elems_ = 0;

@@ -32,6 +32,19 @@ template <class T>
 void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
                  const int* d_split_ids, int nsplit, int bm, hipStream_t s, bool mfma = false);
 
+/* The factorization's updates C -= A B (op(A) = A, alpha = +-1, beta = 1)
+ * of rank K <= GEMM_UPDATE_KMAX and at least 128 columns, streamed on the
+ * matrix cores (bit-identical to launch_gemm with mfma).  A workgroup takes
+ * gemm_update_rows(maxk) rows of C; gemm_update_plan numbers the problems'
+ * workgroups (tile0) for that row count and returns their total. */
+constexpr int GEMM_UPDATE_KMAX = 64;
+int gemm_update_variant(int maxk);
+int gemm_update_rows(int maxk);
+bool gemm_update_eligible(const GemmDescT<double>& g);
+int gemm_update_plan(std::vector<GemmDescT<double>>& descs, int rows);
+void launch_gemm_update(const GemmDescT<double>* d_descs, int count, int nwg, int maxk,
+                        hipStream_t s);
+
 }  // namespace hodlr
 
 #endif

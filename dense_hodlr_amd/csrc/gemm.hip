@@ -339,6 +339,115 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       }
 }
 
+/* The factorization's low-rank updates C -= A B (op(A) = A, alpha = -1 or
+ * +1 and beta = 1, K <= 4 KS) on the matrix cores, streamed: a workgroup
+ * takes 64 RB rows of C and UPD_NCH consecutive 64-column tiles.  Its slice
+ * of A (64 RB x K) is loaded once into registers as the instruction's
+ * operands; per column tile the K x 64 tile of B is staged through LDS, and
+ * the next tile's C and B values are loaded while the current tile is
+ * multiplied, so the read-modify-write of C streams instead of paying a
+ * tile's latency.  Wave w computes rows 16 RB w .. 16 RB w + 16 RB - 1 of
+ * the tile as RB x 4 blocks of 16 x 16, in the layout of gemm_mfma_kernel;
+ * the accumulation (C scaled by alpha, then the K steps of four in
+ * increasing order, padded to a multiple of 16 as there) is that kernel's,
+ * so the results are bit-identical. */
+constexpr int UPD_NCH = 8;
+
+template <int KS, int RB>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+    gemm_update_kernel(const GemmDescT<double>* __restrict__ descs, int count) {
+  constexpr int WR = 64 * RB;  // rows per workgroup
+  __shared__ double Bs[4 * KS][64 + 2];
+  const int wgl = xcd_tile();
+  const GemmDescT<double> g = descs[find_problem(descs, count, wgl)];
+  const int tm = (g.M + WR - 1) / WR, ntn = (g.N + 63) / 64;
+  const int w = wgl - g.tile0, mb = w % tm, nchunk = w / tm;
+  const int m0 = mb * WR, t0 = nchunk * UPD_NCH, t1 = min(ntn, t0 + UPD_NCH);
+  const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+  const int lr = lane & 15, lk = lane >> 4;
+  const int r0 = m0 + 16 * RB * wave;
+  const int nks = (g.K + 15) / 16 * 4;  // K steps of four, as in gemm_mfma_kernel's slices
+  const double alpha = g.alpha;
+  // A as operands: av[a][s] = A(r0 + 16 a + lr, 4 s + lk)
+  double av[RB][KS];
+#pragma unroll
+  for (int a = 0; a < RB; a++)
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      const int gm = r0 + 16 * a + lr, gk = 4 * s + lk;
+      av[a][s] = (gm < g.M && gk < g.K) ? g.A[(size_t)gm + (size_t)gk * g.lda] : 0.0;
+    }
+  // C of column tile tt: cn[a][b][i] = C(r0 + 16 a + lr, 64 tt + 16 b + lk + 4 i)
+  double cn[RB][4][4];
+  auto load_c = [&](int tt) {
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int gm = r0 + 16 * a + lr, gn = 64 * tt + 16 * b + lk + 4 * i;
+          cn[a][b][i] = (gm < g.M && gn < g.N) ? g.C[(size_t)gm + (size_t)gn * g.ldc] : 0.0;
+        }
+  };
+  // B of column tile tt, this thread's share: rows e % (4 KS), column e / (4 KS)
+  double rb[KS];
+  auto load_b = [&](int tt) {
+#pragma unroll
+    for (int q = 0; q < KS; q++) {
+      const int e = t + 256 * q, kk = e % (4 * KS), nn = e / (4 * KS);
+      const int gn = 64 * tt + nn;
+      rb[q] = (kk < g.K && gn < g.N) ? g.B[(size_t)kk + (size_t)gn * g.ldb] : 0.0;
+    }
+  };
+  if (t0 < t1) {
+    load_c(t0);
+    load_b(t0);
+  }
+  for (int tt = t0; tt < t1; tt++) {
+    __syncthreads();  // every wave is done with the previous tile of B
+#pragma unroll
+    for (int q = 0; q < KS; q++) {
+      const int e = t + 256 * q;
+      Bs[e % (4 * KS)][e / (4 * KS)] = rb[q];
+    }
+    v4d acc[RB][4];
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) acc[a][b][i] = alpha * cn[a][b][i];
+    __syncthreads();
+    if (tt + 1 < t1) {
+      load_c(tt + 1);
+      load_b(tt + 1);
+    }
+#pragma unroll
+    for (int s = 0; s < KS; s++) {
+      if (s < nks) {
+        double bv[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) bv[b] = Bs[4 * s + lk][16 * b + lr];
+#pragma unroll
+        for (int a = 0; a < RB; a++)
+#pragma unroll
+          for (int b = 0; b < 4; b++)
+            acc[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(bv[b], av[a][s], acc[a][b], 0, 0, 0);
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < RB; a++)
+#pragma unroll
+      for (int b = 0; b < 4; b++)
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int gm = r0 + 16 * a + lr, gn = 64 * tt + 16 * b + lk + 4 * i;
+          if (gm < g.M && gn < g.N) g.C[(size_t)gm + (size_t)gn * g.ldc] = alpha * acc[a][b][i];
+        }
+  }
+}
+
 template <class T>
 __global__ void __launch_bounds__(256)
     splitk_reduce_kernel(const GemmDescT<T>* __restrict__ descs, const int* __restrict__ ids,
@@ -447,6 +556,37 @@ void launch_gemm(const GemmDescT<T>* d_descs, int count, int ntiles, T* d_ws,
     splitk_reduce_kernel<T><<<g, 256, 0, s>>>(d_descs, d_split_ids, d_ws);
     HODLR_HIP_CHECK(hipGetLastError());
   }
+}
+
+int gemm_update_variant(int maxk) { return maxk <= 32 ? 0 : maxk <= 48 ? 1 : 2; }
+
+int gemm_update_rows(int maxk) { return gemm_update_variant(maxk) == 0 ? 128 : 64; }
+
+bool gemm_update_eligible(const GemmDesc& g) {
+  return g.trans_a == 0 && g.beta == 1.0 && (g.alpha == 1.0 || g.alpha == -1.0) &&
+         g.K <= GEMM_UPDATE_KMAX && g.N >= 128;
+}
+
+int gemm_update_plan(std::vector<GemmDesc>& descs, int rows) {
+  int wg = 0;
+  for (auto& g : descs) {
+    const int tm = (g.M + rows - 1) / rows, ntn = (g.N + 63) / 64;
+    g.ksplit = 1;
+    g.ws_off = 0;
+    g.tile0 = wg;
+    wg += tm * ((ntn + UPD_NCH - 1) / UPD_NCH);
+  }
+  return wg;
+}
+
+void launch_gemm_update(const GemmDesc* d_descs, int count, int nwg, int maxk, hipStream_t s) {
+  if (count == 0 || nwg == 0) return;
+  switch (gemm_update_variant(maxk)) {
+    case 0: gemm_update_kernel<8, 2><<<nwg, 256, 0, s>>>(d_descs, count); break;
+    case 1: gemm_update_kernel<12, 1><<<nwg, 256, 0, s>>>(d_descs, count); break;
+    default: gemm_update_kernel<16, 1><<<nwg, 256, 0, s>>>(d_descs, count);
+  }
+  HODLR_HIP_CHECK(hipGetLastError());
 }
 
 template int gemm_tile_m<double>(const std::vector<GemmDesc>&, bool);

@@ -200,6 +200,34 @@ def test_root_lu_side_stream_matches_in_order(monkeypatch):
     assert np.array_equal(out[0][0], out[0][1])
 
 
+def test_streamed_updates_match_tiled(monkeypatch):
+    """The factorization's small-rank updates W -= W' Y (rank <= 64, at least
+    1024 workgroups) go through the streaming matrix-core kernel, which keeps
+    its rows of W' in registers and prefetches the next column tile of W;
+    switched off (HODLR_NO_STREAM_UPDATE) they go through the tiled kernel.
+    The accumulation order is the same, so recSM and recLU solutions and the
+    log-det are bit-identical (65536 points in 2D, leaf 128: the deepest
+    levels' updates have 1024 workgroups and more)."""
+    _case("streamed vs tiled small-rank updates")
+    n = 65536
+    pts = I.points_2d_kd(n, 71)
+    b = I.rhs_normal(n, 2, 72)
+    out = []
+    for flag in (None, "1"):
+        if flag:
+            monkeypatch.setenv("HODLR_NO_STREAM_UPDATE", flag)
+        else:
+            monkeypatch.delenv("HODLR_NO_STREAM_UPDATE", raising=False)
+        g = HODLR_Matrix.from_kernel(pts, "gaussian", 0.05, 1e-3, sizeThreshold=128)
+        g.set_LRTolerance(1e-8)
+        out.append((g.recSM_Solve(b), g.recLU_Solve(b), g.logAbsDeterminant(), g.nodes()))
+        del g
+    assert np.array_equal(out[0][3], out[1][3])
+    assert max(np.max(out[0][3][out[0][3][:, 0] >= 6][:, 4:]), 1) <= 64  # deep ranks stream
+    for u, v in zip(out[0][:3], out[1][:3]):
+        assert np.array_equal(np.asarray(u), np.asarray(v))
+
+
 @pytest.mark.parametrize("n", [3000, 4500])
 def test_single_large_leaf_lu_paths(n):
     """One dense leaf of n rows (sizeThreshold above n): the batched LU at
