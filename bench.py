@@ -524,8 +524,10 @@ def main():
                          "bytes": q["bytes"], "launches": q["launches"],
                          "roofline_time_s": q["bound_time"],
                          "frac_of_roofline_time": q["bound_time"] / q["time"],
-                         "frac_of_measured_fp64": tf / (peaks["fp64_mfma_tflops"] if key == "gemm"
-                                                        else peaks["fp64_valu_tflops"])})
+                         # (the batched LU runs on the vector units, the rest on the
+                         # matrix cores with the default launch shapes)
+                         "frac_of_measured_fp64": tf / (peaks["fp64_valu_tflops"] if key == "lu"
+                                                        else peaks["fp64_mfma_tflops"])})
         out["roofline_factorization"] = more
         del g
 
