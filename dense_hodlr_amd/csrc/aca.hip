@@ -209,16 +209,30 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
         if (dots) red[ll][t] = p;
       }
     } else {
-      for (int ll = 0; ll < lc; ll++) {
-        const double b = RES ? bc[(size_t)(l0 + ll) * ld] : 0.0;
-        double p = 0.0;
+      // the last, partial chunk: four columns at a time, their loads issued
+      // together (one memory round trip per four columns, not per column)
+      for (int l4 = 0; l4 < lc; l4 += 4) {
+        double v4[4][JP], b4[4];
 #pragma unroll
-        for (int q = 0; q < JP; q++) {
-          const double v = idx[q] >= 0 ? P[(size_t)idx[q] + (size_t)(l0 + ll) * ld] : 0.0;
-          if (RES) s[q] = add_mul_rn(s[q], b, v);
-          p += v * pv[q];
+        for (int u = 0; u < 4; u++) {
+          const bool ok = l4 + u < lc;
+          if (RES) b4[u] = ok ? bc[(size_t)(l0 + l4 + u) * ld] : 0.0;
+#pragma unroll
+          for (int q = 0; q < JP; q++)
+            v4[u][q] = (ok && idx[q] >= 0) ? P[(size_t)idx[q] + (size_t)(l0 + l4 + u) * ld] : 0.0;
         }
-        if (dots) red[ll][t] = p;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+          if (l4 + u < lc) {
+            double p = 0.0;
+#pragma unroll
+            for (int q = 0; q < JP; q++) {
+              if (RES) s[q] = add_mul_rn(s[q], b4[u], v4[u][q]);
+              p += v4[u][q] * pv[q];
+            }
+            if (dots) red[l4 + u][t] = p;
+          }
+        }
       }
     }
     if (dots) {
