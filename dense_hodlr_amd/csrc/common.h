@@ -34,6 +34,21 @@ struct NotConverged : std::runtime_error {
                               " in " + #expr);                                         \
   } while (0)
 
+/* Pointers read from descriptors are generic (flat) to the compiler, and
+ * flat memory instructions count against the LDS wait counter as well: a
+ * wait for an LDS read then also waits for every global load or store in
+ * flight (a register prefetch, a store before a barrier).  Every device
+ * buffer of the engine is global memory; kernels re-type the descriptor
+ * pointers they access with to_global. */
+#ifdef __HIPCC__
+template <class T>
+using gptr = T __attribute__((address_space(1)))*;
+template <class T>
+__device__ __forceinline__ gptr<T> to_global(T* p) {
+  return (gptr<T>)p;
+}
+#endif
+
 /* Source of matrix entries for one instance (problem) of a forest. */
 struct EntrySource {
   int kind;            /* 0 typed kernel, 1 dense column-major matrix */

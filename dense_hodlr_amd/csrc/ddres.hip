@@ -82,6 +82,7 @@ __global__ void __launch_bounds__(TPB)
   __shared__ double Vs[PR][CT + 1];
   __shared__ double Xs[PR][JT + 1];
   const DdProjDesc d = descs[find_desc(descs, count, (int)blockIdx.x, &DdProjDesc::wg0)];
+  const gptr<const double> dV = to_global(d.V), dX = to_global(d.X);
   const int nct = (d.k + CT - 1) / CT, njt = (d.nrhs + JT - 1) / JT;
   int w = (int)blockIdx.x - d.wg0;
   const int chunk = w / (nct * njt);
@@ -103,14 +104,14 @@ __global__ void __launch_bounds__(TPB)
     for (int q = 0; q < VL; q++) {
       const int e = t + q * TPB;
       const int rr = e % PR, cc = e / PR, gr = r0 + rr;
-      rv[q] = (gr < rend && c0 + cc < d.k) ? d.V[(size_t)gr + (size_t)(c0 + cc) * d.ldv] : 0.0;
+      rv[q] = (gr < rend && c0 + cc < d.k) ? dV[(size_t)gr + (size_t)(c0 + cc) * d.ldv] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < XL; q++) {
       const int e = t + q * TPB;
       const int rr = e % PR, cc = e / PR, gr = r0 + rr;
       rx[q] = (e < PR * JT && gr < rend && j0 + cc < d.nrhs)
-                  ? d.X[(size_t)gr + (size_t)(j0 + cc) * d.ldx]
+                  ? dX[(size_t)gr + (size_t)(j0 + cc) * d.ldx]
                   : 0.0;
     }
   };
@@ -202,6 +203,8 @@ __global__ void __launch_bounds__(TPB)
   __shared__ double Th[AK][AJ];
   __shared__ double Tl[AK][AJ];
   const DdApplyDesc d = descs[find_desc(descs, count, (int)blockIdx.x, &DdApplyDesc::wg0)];
+  const gptr<const double> tH = to_global(d.t_hi), tL = to_global(d.t_lo);
+  const gptr<double> yH = to_global(d.y_hi), yL = to_global(d.y_lo);
   const int nrt = (d.rows + TPB - 1) / TPB;
   const int w = (int)blockIdx.x - d.wg0;
   const int row0 = (w % nrt) * TPB, j0 = (w / nrt) * AJ;
@@ -222,13 +225,13 @@ __global__ void __launch_bounds__(TPB)
         const int cc = e % AK, jj = e / AK;
         const bool ok = cc < nc && jj < nj;
         const size_t o = (size_t)(c0 + cc) + (size_t)(j0 + jj) * d.ldt;
-        Th[cc][jj] = ok ? d.t_hi[o] : 0.0;
-        Tl[cc][jj] = (ok && d.t_lo) ? d.t_lo[o] : 0.0;
+        Th[cc][jj] = ok ? tH[o] : 0.0;
+        Tl[cc][jj] = (ok && d.t_lo) ? tL[o] : 0.0;
       }
     }
     __syncthreads();
     if (row < d.rows) {
-      const double* a = d.A + row + (size_t)c0 * d.lda;
+      const gptr<const double> a = to_global(d.A) + row + (size_t)c0 * d.lda;
       for (int cc = 0; cc < nc; cc++) {
         const double av = a[(size_t)cc * d.lda];
 #pragma unroll
@@ -245,14 +248,14 @@ __global__ void __launch_bounds__(TPB)
     double h = sh[jj], l = sl[jj];
     if (d.accumulate) {
       double s, q;
-      two_sum(d.y_hi[o], h, s, q);
+      two_sum(yH[o], h, s, q);
       h = s;
-      l = q + (l + d.y_lo[o]);
+      l = q + (l + yL[o]);
     }
     double hh, ll;
     two_sum(h, l, hh, ll);
-    d.y_hi[o] = hh;
-    d.y_lo[o] = ll;
+    yH[o] = hh;
+    yL[o] = ll;
   }
 }
 

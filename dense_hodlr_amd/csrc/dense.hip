@@ -206,6 +206,53 @@ __global__ void __launch_bounds__(TPB) lu_panel_kernel(const LuDesc* __restrict_
   }
 }
 
+/* (v, i) <- the wave's first maximum (largest v, smallest i among equal v),
+ * in every lane.  Within rows of 16 lanes by DPP moves (pairs swapped,
+ * pairs of pairs swapped, half-row mirror, row mirror: after each step a
+ * lane holds the best of a group twice as large), then the four rows'
+ * results through readlane.  The order of the comparisons does not matter:
+ * "first maximum" is a total order on distinct indices. */
+template <int CTRL>
+__device__ __forceinline__ void first_max_step(double& v, int& i) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  const int oi = __builtin_amdgcn_update_dpp(0, i, CTRL, 0xF, 0xF, false);
+  const double ov =
+      __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+  if (ov > v || (ov == v && oi < i)) {
+    v = ov;
+    i = oi;
+  }
+}
+
+__device__ __forceinline__ double read_lane_f64(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ void wave_first_max(double& v, int& i) {
+  first_max_step<0xB1>(v, i);   // quad_perm [1, 0, 3, 2]
+  first_max_step<0x4E>(v, i);   // quad_perm [2, 3, 0, 1]
+  first_max_step<0x141>(v, i);  // row_half_mirror
+  first_max_step<0x140>(v, i);  // row_mirror
+  double wv = read_lane_f64(v, 0);
+  int wi = __builtin_amdgcn_readlane(i, 0);
+#pragma unroll
+  for (int r = 16; r < 64; r += 16) {
+    const double ov = read_lane_f64(v, r);
+    const int oi = __builtin_amdgcn_readlane(i, r);
+    if (ov > wv || (ov == wv && oi < wi)) {
+      wv = ov;
+      wi = oi;
+    }
+  }
+  v = wv;
+  i = wi;
+}
+
 /* The same panel step with the panel held in registers: thread t owns rows
  * p0 + t + NT q (q < RPT) of the panel columns [p0, p0 + pw), pw <= PW.
  * Per column: first-max pivot search over the owned rows (wave shuffles,
@@ -231,7 +278,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
   const LuDesc m = d[blockIdx.x];
   const int n = m.n, ld = m.ld, t = threadIdx.x;
   if (n <= p0) return;
-  double* A = m.A;
+  const gptr<double> A = to_global(m.A);
   const int pw = min(min(nb, PW), n - p0);
   double R[RPT][PW];
 #pragma unroll
@@ -241,10 +288,20 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
     for (int j = 0; j < PW; j++)
       R[q][j] = (r < n && j < pw) ? A[(size_t)r + (size_t)(p0 + j) * ld] : 0.0;
   }
-#pragma unroll
-  for (int c = 0; c < PW; c++) {
-    if (c < pw) {  // (uniform; a guard rather than a break keeps the loop unrolled)
+  // The column loop is not unrolled (a fully unrolled 32-column panel is
+  // ~20k instructions, streamed through the instruction cache once per
+  // column): the uniform column index c selects registers by a compare
+  // chain instead of by register indexing.
+  for (int c = 0; c < pw; c++) {
     const int cr = p0 + c, buf = c & 1;
+    double rc[RPT];  // this thread's entries of column c
+#pragma unroll
+    for (int q = 0; q < RPT; q++) {
+      double x = R[q][0];
+#pragma unroll
+      for (int j = 1; j < PW; j++) x = (j == c) ? R[q][j] : x;
+      rc[q] = x;
+    }
     // first maximum of |A(i, cr)|, i >= cr
     double bv = -1.0;
     int bi = n;
@@ -252,21 +309,14 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
     for (int q = 0; q < RPT; q++) {
       const int r = p0 + t + NT * q;
       if (r >= cr && r < n) {
-        const double v = fabs(R[q][c]);
+        const double v = fabs(rc[q]);
         if (v > bv) {
           bv = v;
           bi = r;
         }
       }
     }
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(bv, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov > bv || (ov == bv && oi < bi)) {
-        bv = ov;
-        bi = oi;
-      }
-    }
+    wave_first_max(bv, bi);
     if ((t & 63) == 0) {
       sv[buf][t >> 6] = bv;
       si[buf][t >> 6] = bi;
@@ -280,10 +330,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
         p = si[buf][w];
       }
     if (p >= n) p = cr;  // a column of NaNs has no maximum: keep the row
-    if (t == 0) {
-      m.piv[cr] = p;
-      piv_sh[c] = p;  // read after the loop, behind this column's second barrier
-    }
+    if (t == 0) piv_sh[c] = p;  // read after the loop, behind this column's second barrier
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int r = p0 + t + NT * q;
@@ -306,33 +353,28 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
         } else if (r == p) {
 #pragma unroll
           for (int j = 0; j < PW; j++) R[q][j] = crow[buf][j];
+          rc[q] = crow[buf][c];
         }
       }
     }
+    // rows below the pivot: multiplier l = A(i, cr) / pivot (kept unscaled
+    // for a zero pivot), then A(i, j) -= l A(cr, j) for j > c
     const double pv = prow[buf][c];
-    if (pv != 0.0) {
-      const double inv = 1.0 / pv;
+    const double inv = pv != 0.0 ? 1.0 / pv : 1.0;
+    double pr[PW];
 #pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        const int r = p0 + t + NT * q;
-        if (r > cr && r < n) {
-          R[q][c] *= inv;
+    for (int j = 0; j < PW; j++) pr[j] = prow[buf][j];
 #pragma unroll
-          for (int j = 0; j < PW; j++)
-            if (j > c) R[q][j] -= R[q][c] * prow[buf][j];
+    for (int q = 0; q < RPT; q++) {
+      const int r = p0 + t + NT * q;
+      if (r > cr && r < n) {
+        const double l = pv != 0.0 ? rc[q] * inv : rc[q];
+#pragma unroll
+        for (int j = 0; j < PW; j++) {
+          const double nv = fma(-l, pr[j], R[q][j]);
+          R[q][j] = j > c ? nv : (j == c ? l : R[q][j]);
         }
       }
-    } else {
-#pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        const int r = p0 + t + NT * q;
-        if (r > cr && r < n) {
-#pragma unroll
-          for (int j = 0; j < PW; j++)
-            if (j > c) R[q][j] -= R[q][c] * prow[buf][j];
-        }
-      }
-    }
     }
   }
 #pragma unroll
@@ -361,10 +403,11 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
       }
     }
   __syncthreads();
+  if (t < pw) to_global(m.piv)[p0 + t] = piv_sh[t];
   if (t < 2 * pw) {
     const int dr = t < pw ? p0 + t : piv_sh[t - pw];
-    m.swp[t] = dr;
-    m.swp[LU_SWP_SRC + t] = p0 + perm[dr - p0];
+    to_global(m.swp)[t] = dr;
+    to_global(m.swp)[LU_SWP_SRC + t] = p0 + perm[dr - p0];
   }
 }
 
@@ -390,6 +433,7 @@ __global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* _
       hi = mid - 1;
   }
   const LuStepDesc p = d[lo];
+  const gptr<double> pA = to_global(p.A);
   const int t = threadIdx.x, pw = p.pw, p0 = p.p0;
   const int ch = (int)blockIdx.x - p.chunk0;
   const bool left = ch < p.nleft;
@@ -397,33 +441,33 @@ __global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* _
   const int jend = left ? p0 : p.n;
   const int nm = swap ? 2 * pw : 0;  // (recorded only by the register panels: pw <= 32)
   for (int u = t; u < nm; u += LU_STEP_NT) {
-    sd[u] = p.swp[u];
-    ss[u] = p.swp[LU_SWP_SRC + u];
+    sd[u] = to_global(p.swp)[u];
+    ss[u] = to_global(p.swp)[LU_SWP_SRC + u];
   }
   if (!left)
     for (int e = t; e < pw * pw; e += LU_STEP_NT) {
       const int r = e % pw, c = e / pw;
-      T[r][c] = p.A[(size_t)(p0 + r) + (size_t)(p0 + c) * p.ld];
+      T[r][c] = pA[(size_t)(p0 + r) + (size_t)(p0 + c) * p.ld];
     }
   __syncthreads();
   if (nm > 0) {
     for (int e = t; e < nm * 64; e += LU_STEP_NT) {
       const int u = e >> 6, c = e & 63;
       if (j0 + c < jend && ss[u] != sd[u])
-        X[u][c] = p.A[(size_t)ss[u] + (size_t)(j0 + c) * p.ld];
+        X[u][c] = pA[(size_t)ss[u] + (size_t)(j0 + c) * p.ld];
     }
     __syncthreads();
     for (int e = t; e < nm * 64; e += LU_STEP_NT) {
       const int u = e >> 6, c = e & 63;
       if (j0 + c < jend && ss[u] != sd[u])
-        p.A[(size_t)sd[u] + (size_t)(j0 + c) * p.ld] = X[u][c];
+        pA[(size_t)sd[u] + (size_t)(j0 + c) * p.ld] = X[u][c];
     }
     __syncthreads();  // the substitution below reads rows the stores wrote
   }
   if (left || t >= 64) return;
   const int j = j0 + t;
   const bool ok = j < jend;
-  double* col = p.A + (size_t)j * p.ld;
+  const gptr<double> col = pA + (size_t)j * p.ld;
   for (int r = 0; r < pw; r++) X[r][t] = ok ? col[p0 + r] : 0.0;
   for (int r = 1; r < pw; r++) {
     double v = X[r][t];
@@ -743,6 +787,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
       hi = mid - 1;
   }
   const SolveDesc p = d[lo];
+  const gptr<double> pB = to_global(p.B);
   const int n = p.n, t = threadIdx.x;
   const int c0 = ((int)blockIdx.x - p.chunk0) * CB;
   const int nc = min(CB, p.ncols - c0);
@@ -750,7 +795,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
   // by all threads, replayed from LDS by one)
   for (int i = t; i < n; i += TPB) {
     perm[i] = i;
-    pv[i] = p.piv[i];
+    pv[i] = to_global(p.piv)[i];
   }
   __syncthreads();
   if (t == 0)
@@ -769,9 +814,9 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
     const int i = t + TPB * r;
 #pragma unroll
     for (int c = 0; c < CB; c++)
-      x[r][c] = (i < n && c < nc) ? p.B[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
+      x[r][c] = (i < n && c < nc) ? pB[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
   }
-  const double* LU = p.LU;
+  const gptr<const double> LU = to_global(p.LU);
   const size_t ld = (size_t)p.ld_lu;
   double fr[D][R];
   // forward: unit lower triangle; fr[s] holds column l (rows below the
@@ -864,7 +909,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_small_kernel(const SolveDesc* __
     if (i < n)
 #pragma unroll
       for (int c = 0; c < CB; c++)
-        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+        if (c < nc) pB[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
   }
 }
 
@@ -911,6 +956,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
       hi = mid - 1;
   }
   const SolveDesc p = d[lo];
+  const gptr<double> pB = to_global(p.B);
   const int n = p.n;
   const int c0 = (unit - p.chunk0) * CB;
   const int nc = min(CB, p.ncols - c0);
@@ -921,7 +967,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
   for (int r = 0; r < R; r++) {
     const int i = lane + 64 * r;
     pm[r] = i;
-    pvr[r] = i < n ? p.piv[i] : i;
+    pvr[r] = i < n ? to_global(p.piv)[i] : i;
   }
 #pragma unroll
   for (int g = 0; g < R; g++)
@@ -947,9 +993,9 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
     const int i = lane + 64 * r;
 #pragma unroll
     for (int c = 0; c < CB; c++)
-      x[r][c] = (i < n && c < nc) ? p.B[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
+      x[r][c] = (i < n && c < nc) ? pB[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
   }
-  const double* LU = p.LU;
+  const gptr<const double> LU = to_global(p.LU);
   const size_t ld = (size_t)p.ld_lu;
   double fr[D][R];
   // forward: unit lower triangle; fr[s] holds column l (rows below the
@@ -1046,7 +1092,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_wave_kernel(const SolveDesc* __r
     if (i < n)
 #pragma unroll
       for (int c = 0; c < CB; c++)
-        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+        if (c < nc) pB[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
   }
 }
 
@@ -1078,6 +1124,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_shared_kernel(const SolveDesc* _
       hi = mid - 1;
   }
   const SolveDesc p = d[lo];
+  const gptr<double> pB = to_global(p.B);
   const int n = p.n;
   const int c0 = ((int)blockIdx.x - p.chunk0) * (4 * CB) + w * CB;
   const int nc = min(CB, p.ncols - c0);  // (<= 0: the wave only takes part in the barriers)
@@ -1086,7 +1133,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_shared_kernel(const SolveDesc* _
   for (int r = 0; r < R; r++) {
     const int i = lane + 64 * r;
     pm[r] = i;
-    pvr[r] = i < n ? p.piv[i] : i;
+    pvr[r] = i < n ? to_global(p.piv)[i] : i;
   }
 #pragma unroll
   for (int g = 0; g < R; g++)
@@ -1112,9 +1159,9 @@ __global__ void __launch_bounds__(TPB) lu_solve_shared_kernel(const SolveDesc* _
     const int i = lane + 64 * r;
 #pragma unroll
     for (int c = 0; c < CB; c++)
-      x[r][c] = (i < n && c < nc) ? p.B[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
+      x[r][c] = (i < n && c < nc) ? pB[(size_t)pm[r] + (size_t)(c0 + c) * p.ldb] : 0.0;
   }
-  const double* LU = p.LU;
+  const gptr<const double> LU = to_global(p.LU);
   const size_t ld = (size_t)p.ld_lu;
   // forward: unit lower triangle, the columns l of row group g in pieces
 #pragma unroll
@@ -1181,7 +1228,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_shared_kernel(const SolveDesc* _
     if (i < n)
 #pragma unroll
       for (int c = 0; c < CB; c++)
-        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+        if (c < nc) pB[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
   }
 }
 
@@ -1210,6 +1257,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
       hi = mid - 1;
   }
   const SolveDesc p = d[lo];
+  const gptr<double> pB = to_global(p.B);
   const int n = p.n, t = threadIdx.x;
   const int c0 = ((int)blockIdx.x - p.chunk0) * CB;
   const int nc = min(CB, p.ncols - c0);
@@ -1217,7 +1265,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
   // by all threads, replayed from LDS by one)
   for (int i = t; i < n; i += TPB) {
     perm[i] = i;
-    pv[i] = p.piv[i];
+    pv[i] = to_global(p.piv)[i];
   }
   __syncthreads();
   if (t == 0)
@@ -1236,9 +1284,9 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
     const int i = t + TPB * r;
 #pragma unroll
     for (int c = 0; c < CB; c++)
-      x[r][c] = (i < n && c < nc) ? p.B[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
+      x[r][c] = (i < n && c < nc) ? pB[(size_t)perm[i] + (size_t)(c0 + c) * p.ldb] : 0.0;
   }
-  const double* LU = p.LU;
+  const gptr<const double> LU = to_global(p.LU);
   const size_t ld = (size_t)p.ld_lu;
   const int lane = t & 63, wave = t >> 6;
   const int nblk = (n + 63) / 64;
@@ -1277,7 +1325,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
     for (int q = 0; q < R; q++) {
       const int i = t + TPB * q;
       if (i >= b0 + bw && i < n) {
-        const double* col = LU + (size_t)i + (size_t)b0 * ld;
+        const gptr<const double> col = LU + (size_t)i + (size_t)b0 * ld;
 #pragma unroll 8
         for (int j = 0; j < bw; j++) {
           const double a = col[(size_t)j * ld];
@@ -1322,7 +1370,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
     for (int q = 0; q < R; q++) {
       const int i = t + TPB * q;
       if (i < b0) {
-        const double* col = LU + (size_t)i + (size_t)b0 * ld;
+        const gptr<const double> col = LU + (size_t)i + (size_t)b0 * ld;
 #pragma unroll 8
         for (int j = bw - 1; j >= 0; j--) {
           const double a = col[(size_t)j * ld];
@@ -1339,7 +1387,7 @@ __global__ void __launch_bounds__(TPB) lu_solve_blk_kernel(const SolveDesc* __re
     if (i < n)
 #pragma unroll
       for (int c = 0; c < CB; c++)
-        if (c < nc) p.B[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
+        if (c < nc) pB[(size_t)i + (size_t)(c0 + c) * p.ldb] = x[r][c];
   }
 }
 
@@ -1391,7 +1439,8 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int n = p.n, nt = (n + 15) >> 4;
   const int q = lane >> 4, c = lane & 15;
   const size_t ld = (size_t)p.ld_lu;
-  const double* __restrict__ F = p.LU;
+  const gptr<const double> F = to_global(p.LU);
+  const gptr<double> pB = to_global(p.B);
   double(*D)[17] = Ds[w];
   // this thread's share of a block column: slice t >> 6, lane t & 63 of
   // every tile: factor row 16 (t0 + u) + (t & 15), column 16 T + 4 (t >> 6)
@@ -1421,11 +1470,11 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
   for (int tt = 0; tt < RT; tt++) {
     const int r = 16 * tt + lr;
-    const int src = r < n ? p.perm[r] : 0;
+    const int src = r < n ? to_global(p.perm)[r] : 0;
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int cc = c0 + lc + i;
-      D[lr][lc + i] = (r < n && cc < p.ncols) ? p.B[(size_t)src + (size_t)cc * p.ldb] : 0.0;
+      D[lr][lc + i] = (r < n && cc < p.ncols) ? pB[(size_t)src + (size_t)cc * p.ldb] : 0.0;
     }
     __builtin_amdgcn_wave_barrier();
 #pragma unroll
@@ -1524,7 +1573,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int i = 0; i < 4; i++) {
       const int cc = c0 + lc + i;
-      if (r < n && cc < p.ncols) p.B[(size_t)r + (size_t)cc * p.ldb] = D[lr][lc + i];
+      if (r < n && cc < p.ncols) pB[(size_t)r + (size_t)cc * p.ldb] = D[lr][lc + i];
     }
   }
 }

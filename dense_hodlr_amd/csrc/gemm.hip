@@ -62,6 +62,8 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
   __shared__ T Bs[2][BK][BN + 1];
   const int gtile = xcd_tile();
   const GemmDescT<T> g = descs[find_problem(descs, count, gtile)];
+  const gptr<const T> gA = to_global(g.A), gB = to_global(g.B);
+  const gptr<T> gC = to_global(g.C);
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   int tile = gtile - g.tile0;
   const int split = tile / (tm * tn);
@@ -92,7 +94,7 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
       }
       const int gm = m0 + mm, gk = k0 + kk;
       ra[r] = (gm < g.M && gk < kend)
-                  ? (g.trans_a ? g.A[(size_t)gk + (size_t)gm * g.lda] : g.A[(size_t)gm + (size_t)gk * g.lda])
+                  ? (g.trans_a ? gA[(size_t)gk + (size_t)gm * g.lda] : gA[(size_t)gm + (size_t)gk * g.lda])
                   : T(0);
     }
 #pragma unroll
@@ -100,7 +102,7 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
       const int e = t + r * NT;
       const int kk = e % BK, nn = e / BK;
       const int gn = n0 + nn, gk = k0 + kk;
-      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : T(0);
+      rb[r] = (gn < g.N && gk < kend) ? gB[(size_t)gk + (size_t)gn * g.ldb] : T(0);
     }
   };
   auto store = [&](int buf) {
@@ -156,7 +158,7 @@ __global__ void __launch_bounds__((BM / 8) * (BN / 4))
       for (int b = 0; b < TN; b++) {
         const int gm = m0 + tx + RX * a, gn = n0 + ty + (BN / TN) * b;
         if (gm < g.M && gn < g.N) {
-          T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+          gptr<T> c = gC + (size_t)gm + (size_t)gn * g.ldc;
           *c = (g.beta == T(0)) ? g.alpha * acc[a][b] : g.alpha * acc[a][b] + g.beta * *c;
         }
       }
@@ -220,6 +222,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
   __shared__ T Bs[BK][BN + 2];
   const int gtile = xcd_tile();
   const GemmDescT<T> g = descs[find_problem(descs, count, gtile)];
+  const gptr<const T> gA = to_global(g.A), gB = to_global(g.B);
+  const gptr<T> gC = to_global(g.C);
   const int tm = (g.M + BM - 1) / BM, tn = (g.N + BN - 1) / BN;
   int tile = gtile - g.tile0;
   const int split = tile / (tm * tn);
@@ -244,7 +248,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       for (int i = 0; i < 4; i++) {
         const int gm = m0 + wm + a * 16 + lr, gn = n0 + wn + b * 16 + Mfma<T>::col(lk, i);
         acc[a][b][i] = (cinit && gm < g.M && gn < g.N)
-                           ? g.alpha * g.C[(size_t)gm + (size_t)gn * g.ldc]
+                           ? g.alpha * gC[(size_t)gm + (size_t)gn * g.ldc]
                            : T(0);
       }
 
@@ -263,7 +267,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       }
       const int gm = m0 + mm, gk = k0 + kk;
       ra[r] = (gm < g.M && gk < kend)
-                  ? (g.trans_a ? g.A[(size_t)gk + (size_t)gm * g.lda] : g.A[(size_t)gm + (size_t)gk * g.lda])
+                  ? (g.trans_a ? gA[(size_t)gk + (size_t)gm * g.lda] : gA[(size_t)gm + (size_t)gk * g.lda])
                   : T(0);
     }
 #pragma unroll
@@ -271,7 +275,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
       const int e = t + r * NT;
       const int kk = e % BK, nn = e / BK;
       const int gn = n0 + nn, gk = k0 + kk;
-      rb[r] = (gn < g.N && gk < kend) ? g.B[(size_t)gk + (size_t)gn * g.ldb] : T(0);
+      rb[r] = (gn < g.N && gk < kend) ? gB[(size_t)gk + (size_t)gn * g.ldb] : T(0);
     }
   };
   auto store = [&]() {
@@ -328,9 +332,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3))
         if (gm < g.M && gn < g.N) {
           const T v = acc[a][b][i];
           if (cinit) {
-            g.C[(size_t)gm + (size_t)gn * g.ldc] = g.alpha * v;
+            gC[(size_t)gm + (size_t)gn * g.ldc] = g.alpha * v;
           } else if (g.ksplit == 1) {
-            T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+            gptr<T> c = gC + (size_t)gm + (size_t)gn * g.ldc;
             *c = (g.beta == T(0)) ? g.alpha * v : g.alpha * v + g.beta * *c;
           } else {
             ws[g.ws_off + (size_t)split * g.M * g.N + (size_t)gm + (size_t)gn * g.M] = v;
@@ -360,6 +364,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ double Bs[4 * KS][64 + 2];
   const int wgl = xcd_tile();
   const GemmDescT<double> g = descs[find_problem(descs, count, wgl)];
+  const gptr<const double> gA = to_global(g.A), gB = to_global(g.B);
+  const gptr<double> gC = to_global(g.C);
   const int tm = (g.M + WR - 1) / WR, ntn = (g.N + 63) / 64;
   const int w = wgl - g.tile0, mb = w % tm, nchunk = w / tm;
   const int m0 = mb * WR, t0 = nchunk * UPD_NCH, t1 = min(ntn, t0 + UPD_NCH);
@@ -375,7 +381,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
     for (int s = 0; s < KS; s++) {
       const int gm = r0 + 16 * a + lr, gk = 4 * s + lk;
-      av[a][s] = (gm < g.M && gk < g.K) ? g.A[(size_t)gm + (size_t)gk * g.lda] : 0.0;
+      av[a][s] = (gm < g.M && gk < g.K) ? gA[(size_t)gm + (size_t)gk * g.lda] : 0.0;
     }
   // C of column tile tt: cn[a][b][i] = C(r0 + 16 a + lr, 64 tt + 16 b + lk + 4 i)
   double cn[RB][4][4];
@@ -387,7 +393,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int gm = r0 + 16 * a + lr, gn = 64 * tt + 16 * b + lk + 4 * i;
-          cn[a][b][i] = (gm < g.M && gn < g.N) ? g.C[(size_t)gm + (size_t)gn * g.ldc] : 0.0;
+          cn[a][b][i] = (gm < g.M && gn < g.N) ? gC[(size_t)gm + (size_t)gn * g.ldc] : 0.0;
         }
   };
   // B of column tile tt, this thread's share: rows e % (4 KS), column e / (4 KS)
@@ -397,7 +403,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int q = 0; q < KS; q++) {
       const int e = t + 256 * q, kk = e % (4 * KS), nn = e / (4 * KS);
       const int gn = 64 * tt + nn;
-      rb[q] = (kk < g.K && gn < g.N) ? g.B[(size_t)kk + (size_t)gn * g.ldb] : 0.0;
+      rb[q] = (kk < g.K && gn < g.N) ? gB[(size_t)kk + (size_t)gn * g.ldb] : 0.0;
     }
   };
   if (t0 < t1) {
@@ -443,7 +449,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
         for (int i = 0; i < 4; i++) {
           const int gm = r0 + 16 * a + lr, gn = 64 * tt + 16 * b + lk + 4 * i;
-          if (gm < g.M && gn < g.N) g.C[(size_t)gm + (size_t)gn * g.ldc] = alpha * acc[a][b][i];
+          if (gm < g.M && gn < g.N) gC[(size_t)gm + (size_t)gn * g.ldc] = alpha * acc[a][b][i];
         }
   }
 }
@@ -453,12 +459,13 @@ __global__ void __launch_bounds__(256)
     splitk_reduce_kernel(const GemmDescT<T>* __restrict__ descs, const int* __restrict__ ids,
                          const T* __restrict__ ws) {
   const GemmDescT<T> g = descs[ids[blockIdx.y]];
+  const gptr<T> gC = to_global(g.C);
   const size_t mn = (size_t)g.M * g.N;
   for (size_t e = (size_t)blockIdx.x * 256 + threadIdx.x; e < mn; e += (size_t)gridDim.x * 256) {
     T s = T(0);
     for (int sp = 0; sp < g.ksplit; sp++) s += ws[g.ws_off + (size_t)sp * mn + e];
     const int gm = (int)(e % g.M), gn = (int)(e / g.M);
-    T* c = g.C + (size_t)gm + (size_t)gn * g.ldc;
+    gptr<T> c = gC + (size_t)gm + (size_t)gn * g.ldc;
     *c = (g.beta == T(0)) ? g.alpha * s : g.alpha * s + g.beta * *c;
   }
 }

@@ -161,6 +161,28 @@ def test_blocked_one_launch_solve_matches_step_kernel(monkeypatch, n):
     assert rel(out[0][0], np.linalg.solve(A, b)) < 1e-13 * np.linalg.cond(A)
 
 
+@pytest.mark.parametrize("n", [200, 500, 1000, 2000, 3000])
+def test_dense_lu_pivots_match_lapack(n):
+    """The batched LU's panels pick the first maximum of each column, as
+    LAPACK's getrf (idamax): random dense matrices of one leaf each, sized
+    for each register-panel variant (256 or 512 threads, 1-8 rows per
+    thread), pivot at almost every column.  determinant() is the product of
+    the leaf LU's diagonal without the interchanges' sign, so it equals
+    LAPACK's product of U's diagonal, sign included, only if every pivot row
+    is the same.  Scaled to |det| = 1."""
+    import scipy.linalg as sla
+    _case(f"dense LU pivots n={n}")
+    for seed in range(3):
+        G = np.random.default_rng(90 + 7 * n + seed).standard_normal((n, n))
+        A = G / np.exp(np.linalg.slogdet(G)[1] / n)
+        dl = float(np.prod(np.diag(sla.lu_factor(A)[0])))
+        g = HODLR_Matrix(A, n + 1, "partialPiv_ACA")
+        dg = g.determinant()
+        del g
+        print(f"n={n} seed={seed} det GPU {dg:.15e} LAPACK {dl:.15e}", flush=True)
+        assert np.sign(dg) == np.sign(dl) and abs(dg - dl) <= 1e-9 * abs(dl)
+
+
 def test_root_lu_side_stream_matches_in_order(monkeypatch):
     """The roots' capacitance LU runs on a second stream while the following
     solve works through the lower levels (the solve waits for it at the
