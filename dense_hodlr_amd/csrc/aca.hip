@@ -156,6 +156,13 @@ __device__ double wg_sum(double v, double* sh) {
   return r;
 }
 
+/* Raw buffer access (CDNA: 32-bit offsets from a 48-bit base, no format
+ * conversion; the resource's fourth word is the gfx9 raw-buffer setting). */
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+__device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+
 /* s + w * v with the product rounded before the sum (no fused multiply-add):
  * the arithmetic of the reference's residual loop.  HIP's __dadd_rn /
  * __dmul_rn are plain operators that the default contraction would fuse. */
@@ -187,54 +194,60 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
                                            double (*wsum)[DOT_WIN], double* out) {
   static_assert(TPB == 256 && LCH == 16, "sixteen parts of sixteen threads per column");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  for (int l0 = 0; l0 < k; l0 += LCH) {
-    const int lc = min(LCH, k - l0);
-    if (lc == LCH) {
-      double vv[LCH][JP], bb[LCH];
+  // Chunks in two halves of H columns, software-pipelined: once a half's
+  // values are used, the same half of the next chunk is loaded into its
+  // registers, so the next chunk's loads are in flight during the rest of
+  // this chunk's arithmetic and its dot-product reduction (one memory round
+  // trip per pass instead of one per chunk).  Columns beyond k (the last
+  // chunk's tail) are neither loaded nor used.
+  constexpr int H = LCH / 2;
+  double va[H][JP], vb[H][JP], ba[H], bb[H];
+  // Loads through buffer resources rebased at each half's first column: the
+  // row offsets stay in four registers and each column's offset is uniform
+  // (a scalar operand), so no 64-bit address is formed or held per load.
+  // Offsets reach (H - 1) ld + ld rows of 8 bytes: ld < 2^25 (checked on
+  // the host).
+  int voff[JP];
 #pragma unroll
-      for (int ll = 0; ll < LCH; ll++) {
-        if (RES) bb[ll] = bc[(size_t)(l0 + ll) * ld];
+  for (int q = 0; q < JP; q++) voff[q] = idx[q] >= 0 ? idx[q] * 8 : 0;
+  auto load_half = [&](double(&v)[H][JP], double(&b)[H], int c0) {
+    const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
+        (void*)(P + (size_t)c0 * ld), (short)0, 0x7fffffff, BUF_RSRC_WORD3);
+    const __amdgpu_buffer_rsrc_t rb =
+        RES ? __builtin_amdgcn_make_buffer_rsrc((void*)(bc + (size_t)c0 * ld), (short)0, 0x7fffffff,
+                                                BUF_RSRC_WORD3)
+            : rp;
 #pragma unroll
-        for (int q = 0; q < JP; q++)
-          vv[ll][q] = idx[q] >= 0 ? P[(size_t)idx[q] + (size_t)(l0 + ll) * ld] : 0.0;
-      }
+    for (int ll = 0; ll < H; ll++) {
+      const bool ok = c0 + ll < k;
+      const int so = ll * ld * 8;
+      if (RES) b[ll] = ok ? buf_load_f64(rb, 0, so) : 0.0;
 #pragma unroll
-      for (int ll = 0; ll < LCH; ll++) {
+      for (int q = 0; q < JP; q++) v[ll][q] = (ok && idx[q] >= 0) ? buf_load_f64(rp, voff[q], so) : 0.0;
+    }
+  };
+  auto use_half = [&](const double(&v)[H][JP], const double(&b)[H], int c0, int off) {
+#pragma unroll
+    for (int ll = 0; ll < H; ll++) {
+      if (c0 + ll < k) {
         double p = 0.0;
 #pragma unroll
         for (int q = 0; q < JP; q++) {
-          if (RES) s[q] = add_mul_rn(s[q], bb[ll], vv[ll][q]);
-          p += vv[ll][q] * pv[q];
+          if (RES) s[q] = add_mul_rn(s[q], b[ll], v[ll][q]);
+          p += v[ll][q] * pv[q];
         }
-        if (dots) red[ll][t] = p;
-      }
-    } else {
-      // the last, partial chunk: four columns at a time, their loads issued
-      // together (one memory round trip per four columns, not per column)
-      for (int l4 = 0; l4 < lc; l4 += 4) {
-        double v4[4][JP], b4[4];
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          const bool ok = l4 + u < lc;
-          if (RES) b4[u] = ok ? bc[(size_t)(l0 + l4 + u) * ld] : 0.0;
-#pragma unroll
-          for (int q = 0; q < JP; q++)
-            v4[u][q] = (ok && idx[q] >= 0) ? P[(size_t)idx[q] + (size_t)(l0 + l4 + u) * ld] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < 4; u++) {
-          if (l4 + u < lc) {
-            double p = 0.0;
-#pragma unroll
-            for (int q = 0; q < JP; q++) {
-              if (RES) s[q] = add_mul_rn(s[q], b4[u], v4[u][q]);
-              p += v4[u][q] * pv[q];
-            }
-            if (dots) red[l4 + u][t] = p;
-          }
-        }
+        if (dots) red[off + ll][t] = p;
       }
     }
+  };
+  load_half(va, ba, 0);
+  load_half(vb, bb, H);
+  for (int l0 = 0; l0 < k; l0 += LCH) {
+    const int lc = min(LCH, k - l0);
+    use_half(va, ba, l0, 0);
+    load_half(va, ba, l0 + LCH);
+    use_half(vb, bb, l0 + H, H);
+    load_half(vb, bb, l0 + LCH + H);
     if (dots) {
       __builtin_amdgcn_wave_barrier();
       // part 4 wave + pp of column ll: threads 64 wave + 16 pp + u, u < 16
@@ -283,7 +296,7 @@ __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, A
 }
 
 template <class SRC, int JPT>
-__global__ void __launch_bounds__(TPB)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
     aca_row_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    const AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
@@ -345,7 +358,7 @@ __global__ void __launch_bounds__(TPB)
 }
 
 template <class SRC, int IPT>
-__global__ void __launch_bounds__(TPB)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
     aca_col_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
