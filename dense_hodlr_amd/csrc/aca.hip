@@ -194,23 +194,26 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
                                            double (*wsum)[DOT_WIN], double* out) {
   static_assert(TPB == 256 && LCH == 16, "sixteen parts of sixteen threads per column");
   const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-  // Chunks in two halves of H columns, software-pipelined: once a half's
-  // values are used, the same half of the next chunk is loaded into its
-  // registers, so the next chunk's loads are in flight during the rest of
-  // this chunk's arithmetic and its dot-product reduction (one memory round
-  // trip per pass instead of one per chunk).  Columns beyond k (the last
-  // chunk's tail) are neither loaded nor used.
-  constexpr int H = LCH / 2;
+  // Columns in pieces of H, software-pipelined over two register buffers:
+  // once a piece's values are used, the piece two ahead is loaded into its
+  // registers, so loads stay in flight during the arithmetic and the
+  // dot-product reductions (one memory round trip per pass instead of one
+  // per chunk).  H keeps the bytes in flight per thread at 8 JP H = 256
+  // whatever the rows per thread: 8 columns for four rows, 16 for one or
+  // two (a piece is then a whole reduction chunk).  Columns beyond k (the
+  // last piece's tail) are neither loaded nor used.
+  constexpr int H = JP >= 4 ? 8 : 16;
+  static_assert(LCH % H == 0, "whole pieces per reduction chunk");
   double va[H][JP], vb[H][JP], ba[H], bb[H];
-  // Loads through buffer resources rebased at each half's first column: the
-  // row offsets stay in four registers and each column's offset is uniform
-  // (a scalar operand), so no 64-bit address is formed or held per load.
-  // Offsets reach (H - 1) ld + ld rows of 8 bytes: ld < 2^25 (checked on
-  // the host).
+  // Loads through buffer resources rebased at each piece's first column:
+  // the row offsets stay in JP registers and each column's offset is
+  // uniform (a scalar operand), so no 64-bit address is formed or held per
+  // load.  Offsets reach H ld rows of 8 bytes: ld < 2^24 (checked on the
+  // host).
   int voff[JP];
 #pragma unroll
   for (int q = 0; q < JP; q++) voff[q] = idx[q] >= 0 ? idx[q] * 8 : 0;
-  auto load_half = [&](double(&v)[H][JP], double(&b)[H], int c0) {
+  auto load_piece = [&](double(&v)[H][JP], double(&b)[H], int c0) {
     const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(P + (size_t)c0 * ld), (short)0, 0x7fffffff, BUF_RSRC_WORD3);
     const __amdgpu_buffer_rsrc_t rb =
@@ -226,7 +229,8 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
       for (int q = 0; q < JP; q++) v[ll][q] = (ok && idx[q] >= 0) ? buf_load_f64(rp, voff[q], so) : 0.0;
     }
   };
-  auto use_half = [&](const double(&v)[H][JP], const double(&b)[H], int c0, int off) {
+  auto use_piece = [&](const double(&v)[H][JP], const double(&b)[H], int c0) {
+    const int off = c0 % LCH;
 #pragma unroll
     for (int ll = 0; ll < H; ll++) {
       if (c0 + ll < k) {
@@ -240,38 +244,45 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
       }
     }
   };
-  load_half(va, ba, 0);
-  load_half(vb, bb, H);
-  for (int l0 = 0; l0 < k; l0 += LCH) {
+  // the dot products of chunk [l0, l0 + LCH), once its pieces are used
+  auto reduce_chunk = [&](int l0) {
     const int lc = min(LCH, k - l0);
-    use_half(va, ba, l0, 0);
-    load_half(va, ba, l0 + LCH);
-    use_half(vb, bb, l0 + H, H);
-    load_half(vb, bb, l0 + LCH + H);
-    if (dots) {
-      __builtin_amdgcn_wave_barrier();
-      // part 4 wave + pp of column ll: threads 64 wave + 16 pp + u, u < 16
-      const int ll = lane >> 2, pp = lane & 3;
-      if (ll < lc) {
-        double sum = 0.0;
+    __builtin_amdgcn_wave_barrier();
+    // part 4 wave + pp of column ll: threads 64 wave + 16 pp + u, u < 16
+    const int ll = lane >> 2, pp = lane & 3;
+    if (ll < lc) {
+      double sum = 0.0;
 #pragma unroll
-        for (int u = 0; u < 16; u++) sum += red[ll][wave * 64 + pp * 16 + u];
-        wsum[wave * 4 + pp][l0 % DOT_WIN + ll] = sum;
-      }
-      __builtin_amdgcn_wave_barrier();
-      if ((l0 + LCH) % DOT_WIN == 0 || l0 + LCH >= k) {
-        __syncthreads();
-        const int wb = l0 / DOT_WIN * DOT_WIN, we = l0 + lc;
-        if (wb + t < we) {
-          const int x = t;
-          double a[8];
+      for (int u = 0; u < 16; u++) sum += red[ll][wave * 64 + pp * 16 + u];
+      wsum[wave * 4 + pp][l0 % DOT_WIN + ll] = sum;
+    }
+    __builtin_amdgcn_wave_barrier();
+    if ((l0 + LCH) % DOT_WIN == 0 || l0 + LCH >= k) {
+      __syncthreads();
+      const int wb = l0 / DOT_WIN * DOT_WIN, we = l0 + lc;
+      if (wb + t < we) {
+        const int x = t;
+        double a[8];
 #pragma unroll
-          for (int q = 0; q < 8; q++) a[q] = wsum[q][x] + wsum[q + 8][x];
-          // the xor tree over parts (offsets 8, 4, 2, 1) as seen by part 0
-          out[wb + x] = ((a[0] + a[4]) + (a[2] + a[6])) + ((a[1] + a[5]) + (a[3] + a[7]));
-        }
-        __syncthreads();
+        for (int q = 0; q < 8; q++) a[q] = wsum[q][x] + wsum[q + 8][x];
+        // the xor tree over parts (offsets 8, 4, 2, 1) as seen by part 0
+        out[wb + x] = ((a[0] + a[4]) + (a[2] + a[6])) + ((a[1] + a[5]) + (a[3] + a[7]));
       }
+      __syncthreads();
+    }
+  };
+  load_piece(va, ba, 0);
+  load_piece(vb, bb, H);
+  for (int c0 = 0; c0 < k; c0 += 2 * H) {
+    use_piece(va, ba, c0);
+    load_piece(va, ba, c0 + 2 * H);
+    if (dots && H == LCH) reduce_chunk(c0);
+    if (c0 + H < k) {
+      use_piece(vb, bb, c0 + H);
+      load_piece(vb, bb, c0 + 3 * H);
+      if (dots) reduce_chunk(H == LCH ? c0 + H : c0);
+    } else if (dots && H < LCH) {
+      reduce_chunk(c0);
     }
   }
 }

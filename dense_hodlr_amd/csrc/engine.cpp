@@ -915,8 +915,8 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
   std::vector<int> cap(nl, 0);
   std::vector<DBuf<double>> P(nl);
   AcaLevels L{};
-  // (the panel passes address 8 columns of a panel by 32-bit byte offsets)
-  if (ntot_ >= (1 << 25)) throw std::invalid_argument("ACA: more than 2^25 - 1 rows in one handle");
+  // (the panel passes address 16 columns of a panel by 32-bit byte offsets)
+  if (ntot_ >= (1 << 24)) throw std::invalid_argument("ACA: more than 2^24 - 1 rows in one handle");
   L.ld = ntot_;
   auto alloc_level = [&](int l, int c) {
     DBuf<double> u, v;
