@@ -280,27 +280,30 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
   if (n <= p0) return;
   const gptr<double> A = to_global(m.A);
   const int pw = min(min(nb, PW), n - p0);
-  double R[RPT][PW];
+  // Rows as vectors: the column c of the loop (uniform, not unrolled) is
+  // read and written by register indexing (one index-mode move per value)
+  // and the rank-1 update runs over the whole row against the pivot row
+  // with its entries at and left of c zeroed, so no per-element selects.
+  // (rows of more than 16 values as several vectors: a larger vector
+  // indexed at run time goes to scratch memory)
+  constexpr int W = PW > 16 ? 16 : PW, NP = PW / W;
+  static_assert(NP <= 2, "at most two vectors per row");
+  typedef double RowVec __attribute__((ext_vector_type(W)));
+  RowVec R[RPT][NP];
 #pragma unroll
   for (int q = 0; q < RPT; q++) {
     const int r = p0 + t + NT * q;
 #pragma unroll
     for (int j = 0; j < PW; j++)
-      R[q][j] = (r < n && j < pw) ? A[(size_t)r + (size_t)(p0 + j) * ld] : 0.0;
+      R[q][j / W][j % W] = (r < n && j < pw) ? A[(size_t)r + (size_t)(p0 + j) * ld] : 0.0;
   }
-  // The column loop is not unrolled (a fully unrolled 32-column panel is
-  // ~20k instructions, streamed through the instruction cache once per
-  // column): the uniform column index c selects registers by a compare
-  // chain instead of by register indexing.
   for (int c = 0; c < pw; c++) {
     const int cr = p0 + c, buf = c & 1;
     double rc[RPT];  // this thread's entries of column c
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
-      double x = R[q][0];
-#pragma unroll
-      for (int j = 1; j < PW; j++) x = (j == c) ? R[q][j] : x;
-      rc[q] = x;
+      rc[q] = R[q][0][c & (W - 1)];
+      if (NP > 1 && c >= W) rc[q] = R[q][NP - 1][c & (W - 1)];
     }
     // first maximum of |A(i, cr)|, i >= cr
     double bv = -1.0;
@@ -336,10 +339,10 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
       const int r = p0 + t + NT * q;
       if (r == p)
 #pragma unroll
-        for (int j = 0; j < PW; j++) prow[buf][j] = R[q][j];
+        for (int j = 0; j < PW; j++) prow[buf][j] = R[q][j / W][j % W];
       if (r == cr)
 #pragma unroll
-        for (int j = 0; j < PW; j++) crow[buf][j] = R[q][j];
+        for (int j = 0; j < PW; j++) crow[buf][j] = R[q][j / W][j % W];
     }
     __syncthreads();
     // exchange rows cr and p of the panel
@@ -349,31 +352,35 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
         const int r = p0 + t + NT * q;
         if (r == cr) {
 #pragma unroll
-          for (int j = 0; j < PW; j++) R[q][j] = prow[buf][j];
+          for (int j = 0; j < PW; j++) R[q][j / W][j % W] = prow[buf][j];
         } else if (r == p) {
 #pragma unroll
-          for (int j = 0; j < PW; j++) R[q][j] = crow[buf][j];
+          for (int j = 0; j < PW; j++) R[q][j / W][j % W] = crow[buf][j];
           rc[q] = crow[buf][c];
         }
       }
     }
     // rows below the pivot: multiplier l = A(i, cr) / pivot (kept unscaled
-    // for a zero pivot), then A(i, j) -= l A(cr, j) for j > c
+    // for a zero pivot), A(i, j) -= l A(cr, j) for j > c, then A(i, cr) = l.
+    // The update runs over the whole row and subtracts l * 0 at j <= c,
+    // which leaves those entries as they are (for a finite l; an entry -0
+    // may turn +0).
     const double pv = prow[buf][c];
     const double inv = pv != 0.0 ? 1.0 / pv : 1.0;
-    double pr[PW];
+    double pm[PW];
 #pragma unroll
-    for (int j = 0; j < PW; j++) pr[j] = prow[buf][j];
+    for (int j = 0; j < PW; j++) pm[j] = j > c ? prow[buf][j] : 0.0;
 #pragma unroll
     for (int q = 0; q < RPT; q++) {
       const int r = p0 + t + NT * q;
       if (r > cr && r < n) {
         const double l = pv != 0.0 ? rc[q] * inv : rc[q];
 #pragma unroll
-        for (int j = 0; j < PW; j++) {
-          const double nv = fma(-l, pr[j], R[q][j]);
-          R[q][j] = j > c ? nv : (j == c ? l : R[q][j]);
-        }
+        for (int j = 0; j < PW; j++) R[q][j / W][j % W] = fma(-l, pm[j], R[q][j / W][j % W]);
+        if (NP > 1 && c >= W)
+          R[q][NP - 1][c & (W - 1)] = l;
+        else
+          R[q][0][c & (W - 1)] = l;
       }
     }
   }
@@ -383,7 +390,7 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
     if (r < n) {
 #pragma unroll
       for (int j = 0; j < PW; j++)
-        if (j < pw) A[(size_t)r + (size_t)(p0 + j) * ld] = R[q][j];
+        if (j < pw) A[(size_t)r + (size_t)(p0 + j) * ld] = R[q][j / W][j % W];
     }
   }
   // The interchanges of the columns outside the panel are applied by
@@ -1614,10 +1621,9 @@ void launch_assemble(const AssembleDesc* d, int count, int max_elems, const Entr
 }
 
 int lu_panel_width(int maxn, int nb) {
-  if (maxn <= 512) return std::min(nb, 32);
-  if (maxn <= 1024) return std::min(nb, 16);
-  if (maxn <= 2048) return std::min(nb, 8);
-  if (maxn <= 4096) return std::min(nb, 4);
+  if (maxn <= 1024) return std::min(nb, 32);
+  if (maxn <= 2048) return std::min(nb, 16);
+  if (maxn <= 4096) return std::min(nb, 8);
   return nb;
 }
 
@@ -1627,20 +1633,19 @@ void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipSt
     lu_panel_reg_kernel<256, 1, 32><<<count, 256, 0, s>>>(d, p0, nb);
   else if (maxn <= 512 && nb <= 32)
     lu_panel_reg_kernel<512, 1, 32><<<count, 512, 0, s>>>(d, p0, nb);
-  else if (maxn <= 1024 && nb <= 16)
-    lu_panel_reg_kernel<512, 2, 16><<<count, 512, 0, s>>>(d, p0, nb);
-  else if (maxn <= 2048 && nb <= 8)
-    lu_panel_reg_kernel<512, 4, 8><<<count, 512, 0, s>>>(d, p0, nb);
-  else if (maxn <= 4096 && nb <= 4)
-    lu_panel_reg_kernel<512, 8, 4><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 1024 && nb <= 32)
+    lu_panel_reg_kernel<512, 2, 32><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 2048 && nb <= 16)
+    lu_panel_reg_kernel<512, 4, 16><<<count, 512, 0, s>>>(d, p0, nb);
+  else if (maxn <= 4096 && nb <= 8)
+    lu_panel_reg_kernel<512, 8, 8><<<count, 512, 0, s>>>(d, p0, nb);
   else
     lu_panel_kernel<<<count, TPB, 0, s>>>(d, p0, nb);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
 bool lu_panel_defers_swaps(int maxn, int nb) {
-  return (maxn <= 512 && nb <= 32) || (maxn <= 1024 && nb <= 16) || (maxn <= 2048 && nb <= 8) ||
-         (maxn <= 4096 && nb <= 4);
+  return (maxn <= 1024 && nb <= 32) || (maxn <= 2048 && nb <= 16) || (maxn <= 4096 && nb <= 8);
 }
 
 void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s) {
