@@ -163,6 +163,13 @@ __device__ __forceinline__ double buf_load_f64(__amdgpu_buffer_rsrc_t r, int vof
   return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
 
+__device__ __forceinline__ double read_lane_f64(double v, int lane) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 /* s + w * v with the product rounded before the sum (no fused multiply-add):
  * the arithmetic of the reference's residual loop.  HIP's __dadd_rn /
  * __dmul_rn are plain operators that the default contraction would fuse. */
@@ -204,7 +211,9 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
   // last piece's tail) are neither loaded nor used.
   constexpr int H = JP >= 4 ? 8 : 16;
   static_assert(LCH % H == 0, "whole pieces per reduction chunk");
-  double va[H][JP], vb[H][JP], ba[H], bb[H];
+  // (RES) the values bc[l ld] of a piece are loaded by one instruction,
+  // lane ll taking column c0 + ll, and broadcast from that lane when used
+  double va[H][JP], vb[H][JP], ba = 0.0, bb = 0.0;
   // Loads through buffer resources rebased at each piece's first column:
   // the row offsets stay in JP registers and each column's offset is
   // uniform (a scalar operand), so no 64-bit address is formed or held per
@@ -213,31 +222,28 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
   int voff[JP];
 #pragma unroll
   for (int q = 0; q < JP; q++) voff[q] = idx[q] >= 0 ? idx[q] * 8 : 0;
-  auto load_piece = [&](double(&v)[H][JP], double(&b)[H], int c0) {
+  auto load_piece = [&](double(&v)[H][JP], double& b, int c0) {
     const __amdgpu_buffer_rsrc_t rp = __builtin_amdgcn_make_buffer_rsrc(
         (void*)(P + (size_t)c0 * ld), (short)0, 0x7fffffff, BUF_RSRC_WORD3);
-    const __amdgpu_buffer_rsrc_t rb =
-        RES ? __builtin_amdgcn_make_buffer_rsrc((void*)(bc + (size_t)c0 * ld), (short)0, 0x7fffffff,
-                                                BUF_RSRC_WORD3)
-            : rp;
+    if (RES) b = (lane < H && c0 + lane < k) ? bc[(size_t)(c0 + lane) * ld] : 0.0;
 #pragma unroll
     for (int ll = 0; ll < H; ll++) {
       const bool ok = c0 + ll < k;
       const int so = ll * ld * 8;
-      if (RES) b[ll] = ok ? buf_load_f64(rb, 0, so) : 0.0;
 #pragma unroll
       for (int q = 0; q < JP; q++) v[ll][q] = (ok && idx[q] >= 0) ? buf_load_f64(rp, voff[q], so) : 0.0;
     }
   };
-  auto use_piece = [&](const double(&v)[H][JP], const double(&b)[H], int c0) {
+  auto use_piece = [&](const double(&v)[H][JP], double b, int c0) {
     const int off = c0 % LCH;
 #pragma unroll
     for (int ll = 0; ll < H; ll++) {
       if (c0 + ll < k) {
+        const double bl = RES ? read_lane_f64(b, ll) : 0.0;
         double p = 0.0;
 #pragma unroll
         for (int q = 0; q < JP; q++) {
-          if (RES) s[q] = add_mul_rn(s[q], b[ll], v[ll][q]);
+          if (RES) s[q] = add_mul_rn(s[q], bl, v[ll][q]);
           p += v[ll][q] * pv[q];
         }
         if (dots) red[off + ll][t] = p;
