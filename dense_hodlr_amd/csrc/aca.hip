@@ -193,7 +193,7 @@ __device__ __forceinline__ double add_mul_rn(double s, double w, double v) {
  * DOT_WIN columns one barrier pair combines the parts of the four waves in a
  * fixed tree.  Parts and tree are those of a 256-thread transposed
  * reduction, so the sums do not depend on the window. */
-template <int JP, bool RES>
+template <int JP, bool RES, bool NARROW>
 __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
                                            const double* __restrict__ bc, int ld, int k,
                                            const int (&idx)[JP], const double (&pv)[JP],
@@ -209,7 +209,7 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
   // whatever the rows per thread: 8 columns for four rows, 16 for one or
   // two (a piece is then a whole reduction chunk).  Columns beyond k (the
   // last piece's tail) are neither loaded nor used.
-  constexpr int H = JP >= 4 ? 8 : 16;
+  constexpr int H = (JP >= 4 ? 8 : 16) / (NARROW ? 2 : 1);
   static_assert(LCH % H == 0, "whole pieces per reduction chunk");
   // (RES) the values bc[l ld] of a piece are loaded by one instruction,
   // lane ll taking column c0 + ll, and broadcast from that lane when used
@@ -282,13 +282,11 @@ __device__ __forceinline__ void panel_pass(const double* __restrict__ P,
   for (int c0 = 0; c0 < k; c0 += 2 * H) {
     use_piece(va, ba, c0);
     load_piece(va, ba, c0 + 2 * H);
-    if (dots && H == LCH) reduce_chunk(c0);
+    if (dots && ((c0 + H) % LCH == 0 || c0 + H >= k)) reduce_chunk(c0 / LCH * LCH);
     if (c0 + H < k) {
       use_piece(vb, bb, c0 + H);
       load_piece(vb, bb, c0 + 3 * H);
-      if (dots) reduce_chunk(H == LCH ? c0 + H : c0);
-    } else if (dots && H < LCH) {
-      reduce_chunk(c0);
+      if (dots && ((c0 + 2 * H) % LCH == 0 || c0 + 2 * H >= k)) reduce_chunk((c0 + H) / LCH * LCH);
     }
   }
 }
@@ -312,8 +310,8 @@ __global__ void __launch_bounds__(TPB) aca_init_kernel(const AcaBlock* blocks, A
   bits[blocks[b].bits_row0] |= 1u;  // row 0 is chosen first (lowRank.cpp:159)
 }
 
-template <class SRC, int JPT>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
+template <class SRC, int JPT, bool NARROW>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(NARROW ? 3 : 2)))
     aca_row_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    const AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
@@ -348,7 +346,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
   }
   double* slot = slot_ptr(L, b, local);
   // residual row and, for the pending term k-1, the dots V(:,l) . v_{k-1}
-  panel_pass<JPT, true>(Vp, Wp + i, ld, k, jj, pv, pend, sacc, red, wsum, slot);
+  panel_pass<JPT, true, NARROW>(Vp, Wp + i, ld, k, jj, pv, pend, sacc, red, wsum, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
@@ -374,8 +372,8 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
   }
 }
 
-template <class SRC, int IPT>
-__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
+template <class SRC, int IPT, bool NARROW>
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(NARROW ? 3 : 2)))
     aca_col_kernel(AcaLevels L, const AcaBlock* __restrict__ blocks, const int* __restrict__ wg2blk,
                    AcaState* __restrict__ st, const uint32_t* __restrict__ bits,
                    const SRC* __restrict__ srcs, int min_pivot) {
@@ -415,7 +413,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
       if (!ok) ii[q] = -1;
       sacc[q] = 0.0;
     }
-    panel_pass<IPT, false>(Wp, nullptr, ld, k, ii, pw, true, sacc, red, wsum, slot);
+    panel_pass<IPT, false, NARROW>(Wp, nullptr, ld, k, ii, pw, true, sacc, red, wsum, slot);
     return;
   }
   const SRC src = srcs[b.inst];
@@ -430,7 +428,7 @@ __global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(2)))
     if (!ok) ii[q] = -1;
     sacc[q] = 0.0;
   }
-  panel_pass<IPT, true>(Wp, Vp + j, ld, k, ii, pw, pend, sacc, red, wsum, slot);
+  panel_pass<IPT, true, NARROW>(Wp, Vp + j, ld, k, ii, pw, pend, sacc, red, wsum, slot);
   PivRed p;
   p.v = -1.0;
   p.i = -1;
@@ -672,18 +670,33 @@ void aca_launch_init(const AcaBlock* d_blocks, AcaState* d_state, uint32_t* d_bi
   HODLR_HIP_CHECK(hipGetLastError());
 }
 
+// Launches of more than ACA_NARROW_WGS workgroups (several per CU) take the
+// variant with half-width pieces: fewer registers, three workgroups per CU
+// instead of two, which hides the latency of the many short workgroups of
+// the early iterations; smaller launches keep the wider pieces, whose loads
+// in flight per wave a lone workgroup per CU needs.  Same arithmetic.
+constexpr int ACA_NARROW_WGS = 768;
+
 template <int JPT>
 static void launch_row(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
                        hipStream_t s) {
-  aca_row_kernel<EntrySource, JPT><<<a.n_row_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_row, a.state,
-                                                             a.bits, d_srcs, a.min_pivot);
+  if (a.n_row_wg > ACA_NARROW_WGS)
+    aca_row_kernel<EntrySource, JPT, true><<<a.n_row_wg, TPB, 0, s>>>(
+        L, a.blocks, a.wg2blk_row, a.state, a.bits, d_srcs, a.min_pivot);
+  else
+    aca_row_kernel<EntrySource, JPT, false><<<a.n_row_wg, TPB, 0, s>>>(
+        L, a.blocks, a.wg2blk_row, a.state, a.bits, d_srcs, a.min_pivot);
 }
 
 template <int IPT>
 static void launch_col(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
                        hipStream_t s) {
-  aca_col_kernel<EntrySource, IPT><<<a.n_col_wg, TPB, 0, s>>>(L, a.blocks, a.wg2blk_col, a.state,
-                                                             a.bits, d_srcs, a.min_pivot);
+  if (a.n_col_wg > ACA_NARROW_WGS)
+    aca_col_kernel<EntrySource, IPT, true><<<a.n_col_wg, TPB, 0, s>>>(
+        L, a.blocks, a.wg2blk_col, a.state, a.bits, d_srcs, a.min_pivot);
+  else
+    aca_col_kernel<EntrySource, IPT, false><<<a.n_col_wg, TPB, 0, s>>>(
+        L, a.blocks, a.wg2blk_col, a.state, a.bits, d_srcs, a.min_pivot);
 }
 
 void aca_launch_iteration(const AcaLevels& L, const AcaLaunch& a, const EntrySource* d_srcs,
