@@ -164,8 +164,9 @@ void launch_lu_panel(const LuDesc* d, int count, int p0, int nb, int maxn, hipSt
  * columns outside the panel to launch_lu_step (register panels; the
  * global-memory panel kernel swaps whole rows itself). */
 bool lu_panel_defers_swaps(int maxn, int nb);
-/* swap: apply the interchanges recorded in LuDesc::swp (see above). */
-void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s);
+/* swap: apply the interchanges recorded in LuDesc::swp (see above); nb: the
+ * batch's panel width (at most 32 when swap). */
+void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, int nb, hipStream_t s);
 void launch_trsm(const TrsmDesc* d, int count, int nwg, hipStream_t s);
 void launch_permute(const PermDesc* d, int count, int nwg, hipStream_t s);
 /* Whole solves of systems of at most LU_SOLVE_SMALL_MAX rows in one launch

@@ -423,13 +423,17 @@ __global__ void __launch_bounds__(NT) lu_panel_reg_kernel(const LuDesc* __restri
  * moves of each column, spread over all four waves: all loads, a barrier,
  * all stores), then for the columns right of the panel U12 = L11^-1 A12 by
  * forward substitution in the first wave, one column per thread, operation
- * for operation as trsm_kernel. */
+ * for operation as trsm_kernel.  PWM is the widest panel: 32 for the
+ * register panels (the only ones with recorded interchanges; the moves are
+ * staged in registers and the LDS holds 25 KB, so six workgroups fit on a
+ * CU), 64 for the global-memory panel (no interchanges here). */
 constexpr int LU_STEP_NT = 256;
 
+template <int PWM>
 __global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* __restrict__ d,
                                                               int count, int swap) {
-  __shared__ double T[64][65];
-  __shared__ double X[64][65];
+  __shared__ double T[PWM][PWM + 1];
+  __shared__ double X[PWM][65];
   __shared__ int sd[2 * 32], ss[2 * 32];
   int lo = 0, hi = count - 1;
   while (lo < hi) {
@@ -446,7 +450,8 @@ __global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* _
   const bool left = ch < p.nleft;
   const int j0 = left ? ch * 64 : p0 + pw + (ch - p.nleft) * 64;  // first column of the chunk
   const int jend = left ? p0 : p.n;
-  const int nm = swap ? 2 * pw : 0;  // (recorded only by the register panels: pw <= 32)
+  // (recorded only by the register panels: pw <= 32)
+  const int nm = (PWM == 32 && swap) ? 2 * pw : 0;
   for (int u = t; u < nm; u += LU_STEP_NT) {
     sd[u] = to_global(p.swp)[u];
     ss[u] = to_global(p.swp)[LU_SWP_SRC + u];
@@ -457,17 +462,22 @@ __global__ void __launch_bounds__(LU_STEP_NT) lu_step_kernel(const LuStepDesc* _
       T[r][c] = pA[(size_t)(p0 + r) + (size_t)(p0 + c) * p.ld];
     }
   __syncthreads();
-  if (nm > 0) {
-    for (int e = t; e < nm * 64; e += LU_STEP_NT) {
-      const int u = e >> 6, c = e & 63;
-      if (j0 + c < jend && ss[u] != sd[u])
-        X[u][c] = pA[(size_t)ss[u] + (size_t)(j0 + c) * p.ld];
+  if (PWM == 32 && nm > 0) {
+    constexpr int MQ = 64 * 64 / LU_STEP_NT;
+    double v[MQ];
+#pragma unroll
+    for (int q = 0; q < MQ; q++) {
+      const int e = t + LU_STEP_NT * q, u = e >> 6, c = e & 63;
+      const bool ok = u < nm && j0 + c < jend && ss[u] != sd[u];
+      v[q] = ok ? pA[(size_t)ss[u] + (size_t)(j0 + c) * p.ld] : 0.0;
     }
+    // every source value is in registers before any row is overwritten
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    for (int e = t; e < nm * 64; e += LU_STEP_NT) {
-      const int u = e >> 6, c = e & 63;
-      if (j0 + c < jend && ss[u] != sd[u])
-        pA[(size_t)sd[u] + (size_t)(j0 + c) * p.ld] = X[u][c];
+#pragma unroll
+    for (int q = 0; q < MQ; q++) {
+      const int e = t + LU_STEP_NT * q, u = e >> 6, c = e & 63;
+      if (u < nm && j0 + c < jend && ss[u] != sd[u]) pA[(size_t)sd[u] + (size_t)(j0 + c) * p.ld] = v[q];
     }
     __syncthreads();  // the substitution below reads rows the stores wrote
   }
@@ -1648,9 +1658,12 @@ bool lu_panel_defers_swaps(int maxn, int nb) {
   return (maxn <= 1024 && nb <= 32) || (maxn <= 2048 && nb <= 16) || (maxn <= 4096 && nb <= 8);
 }
 
-void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, hipStream_t s) {
+void launch_lu_step(const LuStepDesc* d, int count, int nwg, bool swap, int nb, hipStream_t s) {
   if (count == 0 || nwg == 0) return;
-  lu_step_kernel<<<nwg, LU_STEP_NT, 0, s>>>(d, count, swap ? 1 : 0);
+  if (nb <= 32)
+    lu_step_kernel<32><<<nwg, LU_STEP_NT, 0, s>>>(d, count, swap ? 1 : 0);
+  else
+    lu_step_kernel<64><<<nwg, LU_STEP_NT, 0, s>>>(d, count, 0);
   HODLR_HIP_CHECK(hipGetLastError());
 }
 

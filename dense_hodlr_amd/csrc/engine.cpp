@@ -708,7 +708,7 @@ void Engine::lu_batch(std::vector<LuDesc>& descs, hipStream_t s, DescArena& aren
   for (int p0 = 0; p0 < maxn; p0 += nb, p++) {
     launch_lu_panel(dd, (int)descs.size(), p0, nb, maxn, s);
     const Step& st = steps[p];
-    if (st.ntr > 0) launch_lu_step(dtr + st.tr0, st.ntr, st.nwg, defer, s);
+    if (st.ntr > 0) launch_lu_step(dtr + st.tr0, st.ntr, st.nwg, defer, nb, s);
     if (st.ng > 0)
       launch_gemm<double>(dg + st.g0, st.ng, st.ntiles, ws_.p, did ? did + st.id0 : nullptr,
                           st.nid, st.bm, s, tune_.gemm_mfma != 0);
