@@ -66,6 +66,29 @@ def test_factors_match_oracle(gauss1d):
     assert not bad, bad[:10]
 
 
+def test_factors_match_oracle_many_blocks(oracle):
+    """The same bit-for-bit equality when the ACA launches are large enough
+    (more than 768 workgroups: 2046 blocks of a 1024-leaf tree) to take the
+    half-width-piece variant of the panel passes in the early iterations and
+    the full-width one later (the variant is chosen by workgroup count), over
+    many chunks of stored columns: 2D Gaussian N=8192 in kd order, l=0.1,
+    leaf 8 (ranks up to ~300)."""
+    n = 8192
+    pts = I.points_2d_kd(n, 4242)
+    _case("factors gauss2d_8192 leaf 8")
+    g, o = _pair(oracle, n, pts, "gaussian", 0.1, 1e-2, 8, 1e-10)
+    g.storeLRinTree()
+    o.compress()
+    assert np.array_equal(g.nodes(), o.nodes())
+    bad = []
+    for idx in range(len(o.nodes())):
+        for which in range(4):
+            a, b = g.factor(idx, which), o.factor(idx, which)
+            if a.shape != b.shape or not np.array_equal(a, b):
+                bad.append((idx, which))
+    assert not bad, bad[:10]
+
+
 def test_recsm_matches_oracle(gauss1d, oracle):
     cfg, g, o = gauss1d
     _case("recSM gauss1d_4096")
