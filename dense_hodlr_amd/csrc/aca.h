@@ -11,6 +11,14 @@ namespace hodlr {
 constexpr int ACA_MAX_LEVELS = 32;
 constexpr int MAX_FIN_CHUNKS = 256;  // ranks up to 256 * 64 = 16384
 
+/* A block whose row and column phases have at most FIN_PACK_SLOTS
+ * workgroups together is finalized four 64-value chunks per workgroup, a
+ * wave per chunk (aca_fin_kernel); the others one chunk per workgroup. */
+constexpr int FIN_PACK_SLOTS = 32;
+__host__ __device__ inline bool aca_fin_packed(const AcaBlock& b) {
+  return b.row_nwg + b.col_nwg <= FIN_PACK_SLOTS;
+}
+
 /* Device panels of the levels being compressed.  W[l], V[l]: ld x cap[l]
  * column-major; P[l]: per-workgroup partial slots of (cap + 4) doubles. */
 struct AcaLevels {

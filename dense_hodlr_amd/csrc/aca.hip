@@ -477,53 +477,103 @@ __global__ void __launch_bounds__(TPB)
   // column phase); the others leave at once, so early iterations of blocks
   // with large capacity do not pay for their empty chunks
   const int nact = min(nchunk[bidx], s.fin_chunks);
-  if (chunk >= nact) return;
+  // A block with few partial slots packs four chunks into one workgroup, a
+  // wave per chunk (each thread sums its value's slots itself): up to rank
+  // 257 the block then needs one workgroup and no device-scope fence.
+  const bool packed = aca_fin_packed(b);
+  const int nwg = packed ? (nact + 3) / 4 : nact;
+  if (chunk >= nwg) return;
 
   // cross term of the pending term k-1 (lowRank.cpp:234-237): sum_{l<k-1} e_l d_l.
-  // FIN_L values of l per workgroup, each summed over the slots by 4 groups.
+  // FIN_L values of l per chunk, each summed over the slots in 4 groups
+  // (slots w = g mod 4), the groups' sums added in order.
   double cross = 0.0;
-  if (pend) {
-    const int ll = t & (FIN_L - 1), g = t / FIN_L;
-    const int l = chunk * FIN_L + ll;
-    double d = 0.0, e = 0.0;
-    if (l < k - 1) {
-      const size_t stride = (size_t)(L.cap[b.level] + 4);
-      const double* pr = slot_ptr(L, b, 0) + l;
-      const double* pc = slot_ptr(L, b, b.row_nwg) + l;
-      d = sum_strided(pr, g, b.row_nwg, stride);
-      e = sum_strided(pc, g, b.col_nwg, stride);
+  if (packed) {
+    // wave w: chunk 4 chunk + w, with the arithmetic of a one-chunk
+    // workgroup of the unpacked layout (whose other waves hold zeros)
+    const int lane = t & 63, w = t >> 6, lc = 4 * chunk + w;
+    double v = 0.0;
+    if (pend && lc < nact) {
+      const int l = lc * FIN_L + lane;
+      if (l < k - 1) {
+        const size_t stride = (size_t)(L.cap[b.level] + 4);
+        const double* pr = slot_ptr(L, b, 0) + l;
+        const double* pc = slot_ptr(L, b, b.row_nwg) + l;
+        // ((s0 + s1) + s2) + s3 over the four slot groups, as the unpacked
+        // layout adds its groups' sums (a rolled loop: fewer registers)
+        double d = sum_strided(pr, 0, b.row_nwg, stride), e = sum_strided(pc, 0, b.col_nwg, stride);
+#pragma unroll 1
+        for (int g = 1; g < 4; g++) {
+          d = d + sum_strided(pr, g, b.row_nwg, stride);
+          e = e + sum_strided(pc, g, b.col_nwg, stride);
+        }
+        v = d * e;
+      }
     }
-    part[g][ll] = d;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+    // (the unpacked workgroup's final sum over its four waves: this wave's
+    // sum, then the three zero sums of the others)
+    const double z = 0.0;
+    v = (((0.0 + v) + z) + z) + z;
+    if (lane == 0) part[0][w] = v;
     __syncthreads();
-    if (g == 0) d = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
-    __syncthreads();
-    part[g][ll] = e;
-    __syncthreads();
-    if (g == 0) {
-      e = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
-      cross = (l < k - 1) ? d * e : 0.0;
-    }
-  }
-  cross = wg_sum(cross, red);
-  // With one chunk this workgroup is the block's only one: no partials to
-  // publish, no device-scope fences (each writes back this XCD's L2).
-  if (nact > 1) {
-    if (t == 0) {
-      fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = cross;
+    if (nact > 4) {
+      if (t == 0) {
+        for (int q = 0; q < 4 && 4 * chunk + q < nact; q++)
+          fin_part[(size_t)bidx * MAX_FIN_CHUNKS + 4 * chunk + q] = part[0][q];
+        __threadfence();
+        const int old = atomicAdd(&counters[bidx], 1);
+        last = (old == nwg - 1);
+      }
+      __syncthreads();
+      if (!last) return;
       __threadfence();
-      const int old = atomicAdd(&counters[bidx], 1);
-      last = (old == nact - 1);
     }
-    __syncthreads();
-    if (!last) return;
-    // the last workgroup of the block: all its threads reduce, thread 0 decides
-    __threadfence();
+  } else {
+    if (pend) {
+      const int ll = t & (FIN_L - 1), g = t / FIN_L;
+      const int l = chunk * FIN_L + ll;
+      double d = 0.0, e = 0.0;
+      if (l < k - 1) {
+        const size_t stride = (size_t)(L.cap[b.level] + 4);
+        const double* pr = slot_ptr(L, b, 0) + l;
+        const double* pc = slot_ptr(L, b, b.row_nwg) + l;
+        d = sum_strided(pr, g, b.row_nwg, stride);
+        e = sum_strided(pc, g, b.col_nwg, stride);
+      }
+      part[g][ll] = d;
+      __syncthreads();
+      if (g == 0) d = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
+      __syncthreads();
+      part[g][ll] = e;
+      __syncthreads();
+      if (g == 0) {
+        e = part[0][ll] + part[1][ll] + part[2][ll] + part[3][ll];
+        cross = (l < k - 1) ? d * e : 0.0;
+      }
+    }
+    cross = wg_sum(cross, red);
+    // With one chunk this workgroup is the block's only one: no partials to
+    // publish, no device-scope fences (each writes back this XCD's L2).
+    if (nact > 1) {
+      if (t == 0) {
+        fin_part[(size_t)bidx * MAX_FIN_CHUNKS + chunk] = cross;
+        __threadfence();
+        const int old = atomicAdd(&counters[bidx], 1);
+        last = (old == nact - 1);
+      }
+      __syncthreads();
+      if (!last) return;
+      // the last workgroup of the block: all its threads reduce, thread 0 decides
+      __threadfence();
+    }
   }
   const int cap = L.cap[b.level];
   double crossSum = 0.0, nv2 = 0.0, nw2 = 0.0;
   if (pend) {
     for (int c = t; c < nact; c += TPB)
-      crossSum += nact == 1 ? cross : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c];
+      crossSum += packed ? (nact <= 4 ? part[0][c] : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c])
+                         : (nact == 1 ? cross : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c]);
     for (int w = t; w < b.row_nwg; w += TPB) nv2 += slot_ptr(L, b, w)[k - 1];
     for (int w = t; w < b.col_nwg; w += TPB) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
     crossSum = wg_sum(crossSum, red);

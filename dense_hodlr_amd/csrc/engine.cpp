@@ -1022,7 +1022,8 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
       for (int w = 0; w < blk[b].row_nwg; w++) m_row.push_back(b);
       blk[b].col_wg0 = (int)m_col.size();
       for (int w = 0; w < blk[b].col_nwg; w++) m_col.push_back(b);
-      for (int c = 0; c < nchunk[b]; c++) {
+      const int nfin = aca_fin_packed(blk[b]) ? (nchunk[b] + 3) / 4 : nchunk[b];
+      for (int c = 0; c < nfin; c++) {
         m_fin.push_back(b);
         m_chunk.push_back(c);
       }
