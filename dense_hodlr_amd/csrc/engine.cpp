@@ -57,6 +57,8 @@ Engine::Engine(std::vector<InstanceSpec> specs) : specs_(std::move(specs)) {
   HODLR_HIP_CHECK(hipStreamCreateWithFlags(&aux_, hipStreamNonBlocking));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_cap_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_aca_, hipEventDisableTiming));
+  HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_w0_, hipEventDisableTiming));
+  HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_w1_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreateWithFlags(&ev_root_, hipEventDisableTiming));
   HODLR_HIP_CHECK(hipEventCreate(&ev_f0_));
   HODLR_HIP_CHECK(hipEventCreate(&ev_f1_));
@@ -169,6 +171,8 @@ Engine::~Engine() {
   if (ev1_) (void)hipEventDestroy(ev1_);
   if (ev_cap_) (void)hipEventDestroy(ev_cap_);
   if (ev_aca_) (void)hipEventDestroy(ev_aca_);
+  if (ev_w0_) (void)hipEventDestroy(ev_w0_);
+  if (ev_w1_) (void)hipEventDestroy(ev_w1_);
   if (ev_root_) (void)hipEventDestroy(ev_root_);
   if (ev_f0_) (void)hipEventDestroy(ev_f0_);
   if (ev_f1_) (void)hipEventDestroy(ev_f1_);
@@ -1534,14 +1538,20 @@ void Engine::factorize_fp64(int which) {
   spans_.clear();
   span_next_ = 0;
   HODLR_HIP_CHECK(hipEventRecord(ev_f0_, stream_));
-  factor_leaves();
   const size_t N = (size_t)ntot_;
   F.W.alloc(std::max<size_t>(1, N * ktot_));
-  // W starts as a copy of the panels (the reference's U_SM = U, :561-565)
+  // W starts as a copy of the panels (the reference's U_SM = U, :561-565):
+  // the copies (HBM-bound) run on the side stream while the leaves' LU
+  // (latency-bound) runs here; the leaf solves wait for them
+  HODLR_HIP_CHECK(hipEventRecord(ev_w0_, stream_));
+  HODLR_HIP_CHECK(hipStreamWaitEvent(aux_, ev_w0_, 0));
   for (int l = 0; l < nlev_; l++)
     if (K_[l] > 0)
       HODLR_HIP_CHECK(hipMemcpyAsync(F.W.p + N * off_[l], U_[l].p, N * K_[l] * sizeof(double),
-                                     hipMemcpyDeviceToDevice, stream_));
+                                     hipMemcpyDeviceToDevice, aux_));
+  HODLR_HIP_CHECK(hipEventRecord(ev_w1_, aux_));
+  factor_leaves();
+  HODLR_HIP_CHECK(hipStreamWaitEvent(stream_, ev_w1_, 0));
   const int nn = (int)nodes_.size();
   F.c_off.assign(nn, 0);
   F.p_off.assign(nn, 0);

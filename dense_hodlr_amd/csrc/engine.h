@@ -438,6 +438,9 @@ class Engine {
   hipStream_t aux_ = nullptr;
   hipEvent_t ev_cap_ = nullptr, ev_root_ = nullptr;
   hipEvent_t ev_aca_ = nullptr;  // the ACA state snapshot's copy
+  // the factorization's copy of the panels into W on aux_, beside the
+  // leaves' LU on stream_: its start and its end
+  hipEvent_t ev_w0_ = nullptr, ev_w1_ = nullptr;
   // timing of the factorization: its start on stream_, its end on stream_
   // and the end of the roots' LU on aux_ (read by sync_stats)
   hipEvent_t ev_f0_ = nullptr, ev_f1_ = nullptr, ev_froot_ = nullptr;
