@@ -104,7 +104,27 @@ __device__ inline void read_piv(const double* s, int cap, double& v, int& i, int
 }
 
 /* Workgroup-parallel reduction of the pivot partials in slots
- * [s0, s0 + nslot): (max |.|, argmax, first unchosen). */
+ * [s0, s0 + nslot): (max |.|, argmax, first unchosen).  p0 holds slot
+ * s0 + threadIdx.x, already read ((-1, -1, -1) past nslot). */
+__device__ PivRed reduce_slots(const AcaLevels& L, const AcaBlock& b, int s0, int nslot,
+                               PivRed* sh, PivRed p0) {
+  const int cap = L.cap[b.level];
+  PivRed p;
+  p.v = -1.0;
+  p.i = -1;
+  p.first = -1;
+  argmax_merge(p.v, p.i, p0.v, p0.i);
+  if (p0.first >= 0) p.first = p0.first;
+  for (int w = threadIdx.x + TPB; w < nslot; w += TPB) {
+    double v;
+    int i, f;
+    read_piv(slot_ptr(L, b, s0 + w), cap, v, i, f);
+    argmax_merge(p.v, p.i, v, i);
+    if (f >= 0 && (p.first < 0 || f < p.first)) p.first = f;
+  }
+  return wg_reduce_piv(p, sh);
+}
+
 __device__ PivRed reduce_slots(const AcaLevels& L, const AcaBlock& b, int s0, int nslot,
                                PivRed* sh) {
   const int cap = L.cap[b.level];
@@ -132,19 +152,6 @@ __device__ int decide_column(const AcaLevels& L, const AcaBlock& b, const uint32
   return r.first >= 0 ? r.first : -2;
 }
 
-/* sum over w = g, g+4, g+8, ... < n of p[w * stride], eight loads in flight */
-__device__ __forceinline__ double sum_strided(const double* __restrict__ p, int g, int n,
-                                              size_t stride) {
-  double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  int w = g;
-  for (; w + 28 < n; w += 32) {
-#pragma unroll
-    for (int u = 0; u < 8; u++) acc[u] += p[(size_t)(w + 4 * u) * stride];
-  }
-  for (; w < n; w += 4) acc[0] += p[(size_t)w * stride];
-  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-}
-
 /* Deterministic workgroup sum (fixed reduction shape). */
 __device__ double wg_sum(double v, double* sh) {
   for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
@@ -168,6 +175,62 @@ __device__ __forceinline__ double read_lane_f64(double v, int lane) {
   const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, lane);
   const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
   return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+/* Strided slot sums: S(p, g, n) = sum over w = g, g+4, g+8, ... < n of
+ * p[w * stride], as eight accumulators: value m (w = g + 4m) of the first
+ * 8 F values, F = the groups of eight that fit whole (w + 28 < n), goes to
+ * accumulator m mod 8, the rest to accumulator 0, each accumulator in
+ * increasing m; then ((a0 + a1) + (a2 + a3)) + ((a4 + a5) + (a6 + a7)).
+ * The loads of a round are all issued before its additions (the values sit
+ * in the caches of other compute units or in HBM: one memory latency per
+ * round instead of one per eight values).  Values past n are +0.0, which
+ * leaves an accumulator unchanged: an accumulator starts at +0.0 and a
+ * round-to-nearest sum is -0.0 only when both terms are.
+ * Loads: buffer resource at the block's first slot, the lane's value offset
+ * in a register, the slot's offset (w * stride bytes, uniform: g and m are
+ * the same across the wave) in a scalar register. */
+__device__ __forceinline__ int strided_full_groups(int n, int g) {
+  return n - g - 29 >= 0 ? (n - g - 29) / 32 + 1 : 0;
+}
+
+/* zero: +0.0 that the compiler cannot fold (the caller's 0.0 * m for an
+ * m >= 0, which for all it knows may be -0.0).  With a value it knows to be
+ * +0.0 for the skipped loads it moves the first addition of each
+ * accumulator (0.0 + v) next to its load, which then waits for that load
+ * before the later ones are issued. */
+template <int M>
+__device__ __forceinline__ void strided_load(__amdgpu_buffer_rsrc_t rs, int voff, int g, int n,
+                                             int stride_b, int m0, double zero, double (&v)[M]) {
+#pragma unroll
+  for (int q = 0; q < M; q++) {
+    const int w = g + 4 * (m0 + q);
+    v[q] = w < n ? buf_load_f64(rs, voff, w * stride_b) : zero;
+  }
+}
+
+/* resource over a block's slots [s0, ...) */
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t slot_rsrc(const AcaLevels& L, const AcaBlock& b,
+                                                            int s0) {
+  return __builtin_amdgcn_make_buffer_rsrc((void*)slot_ptr(L, b, s0), (short)0, 0x7fffffff,
+                                          BUF_RSRC_WORD3);
+}
+
+/* m0 a multiple of 8 */
+template <int M>
+__device__ __forceinline__ void strided_add(const double (&v)[M], int m0, int full,
+                                            double (&acc)[8]) {
+#pragma unroll
+  for (int q = 0; q < M; q++) {
+    if (m0 + q < 8 * full)
+      acc[q & 7] += v[q];
+    else
+      acc[0] += v[q];
+  }
+}
+
+__device__ __forceinline__ double strided_tree(const double (&acc)[8]) {
+  return ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
 }
 
 /* s + w * v with the product rounded before the sum (no fused multiply-add):
@@ -495,20 +558,34 @@ __global__ void __launch_bounds__(TPB)
     double v = 0.0;
     if (pend && lc < nact) {
       const int l = lc * FIN_L + lane;
-      if (l < k - 1) {
-        const size_t stride = (size_t)(L.cap[b.level] + 4);
-        const double* pr = slot_ptr(L, b, 0) + l;
-        const double* pc = slot_ptr(L, b, b.row_nwg) + l;
-        // ((s0 + s1) + s2) + s3 over the four slot groups, as the unpacked
-        // layout adds its groups' sums (a rolled loop: fewer registers)
-        double d = sum_strided(pr, 0, b.row_nwg, stride), e = sum_strided(pc, 0, b.col_nwg, stride);
-#pragma unroll 1
-        for (int g = 1; g < 4; g++) {
-          d = d + sum_strided(pr, g, b.row_nwg, stride);
-          e = e + sum_strided(pc, g, b.col_nwg, stride);
+      const bool on = l < k - 1;
+      const int stride_b = (L.cap[b.level] + 4) * (int)sizeof(double);
+      const int voff = on ? l * (int)sizeof(double) : 0;  // lanes past k - 1: discarded
+      const __amdgpu_buffer_rsrc_t rr = slot_rsrc(L, b, 0), rc = slot_rsrc(L, b, b.row_nwg);
+      const double zero = 0.0 * (double)b.m;  // +0.0 (m >= 0)
+      // at most 32 slots a phase: eight values per slot group, the 32 loads
+      // of a phase issued at once; then ((S0 + S1) + S2) + S3 over the four
+      // groups, as the unpacked layout adds its groups' sums
+      auto phase_sum = [&](__amdgpu_buffer_rsrc_t rs, int n) {
+        double v4[4][8];
+#pragma unroll
+        for (int g = 0; g < 4; g++) strided_load<8>(rs, voff, g, n, stride_b, 0, zero, v4[g]);
+        double r = 0.0;
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+          strided_add<8>(v4[g], 0, strided_full_groups(n, g), acc);
+          r = g == 0 ? strided_tree(acc) : r + strided_tree(acc);
         }
-        v = d * e;
-      }
+        // the sum is complete here: otherwise the compiler moves its
+        // additions past the next phase's loads, holding both phases' 64
+        // values at once (registers for three waves per SIMD instead of seven)
+        asm volatile("" : : "v"(r));
+        return r;
+      };
+      const double d = phase_sum(rr, b.row_nwg);
+      const double e = phase_sum(rc, b.col_nwg);
+      if (on) v = d * e;
     }
     for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
     // (the unpacked workgroup's final sum over its four waves: this wave's
@@ -533,13 +610,29 @@ __global__ void __launch_bounds__(TPB)
     if (pend) {
       const int ll = t & (FIN_L - 1), g = t / FIN_L;
       const int l = chunk * FIN_L + ll;
+      const bool on = l < k - 1;
+      const int stride_b = (L.cap[b.level] + 4) * (int)sizeof(double);
+      const int voff = on ? l * (int)sizeof(double) : 0;  // lanes past k - 1: discarded
+      const __amdgpu_buffer_rsrc_t rr = slot_rsrc(L, b, 0), rc = slot_rsrc(L, b, b.row_nwg);
+      const int gu = __builtin_amdgcn_readfirstlane(g);  // g is one wave's
+      const double zero = 0.0 * (double)b.m;  // +0.0 (m >= 0)
+      // slot group g of both phases, 16 values of each per round (up to
+      // 64 slots a phase in one round)
+      double ad[8] = {0, 0, 0, 0, 0, 0, 0, 0}, ae[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+      const int fd = strided_full_groups(b.row_nwg, gu), fe = strided_full_groups(b.col_nwg, gu);
+      const int nm = (max(b.row_nwg, b.col_nwg) - gu + 3) / 4;
+#pragma unroll 1
+      for (int m0 = 0; m0 < nm; m0 += 16) {
+        double vd[16], ve[16];
+        strided_load<16>(rr, voff, gu, b.row_nwg, stride_b, m0, zero, vd);
+        strided_load<16>(rc, voff, gu, b.col_nwg, stride_b, m0, zero, ve);
+        strided_add<16>(vd, m0, fd, ad);
+        strided_add<16>(ve, m0, fe, ae);
+      }
       double d = 0.0, e = 0.0;
-      if (l < k - 1) {
-        const size_t stride = (size_t)(L.cap[b.level] + 4);
-        const double* pr = slot_ptr(L, b, 0) + l;
-        const double* pc = slot_ptr(L, b, b.row_nwg) + l;
-        d = sum_strided(pr, g, b.row_nwg, stride);
-        e = sum_strided(pc, g, b.col_nwg, stride);
+      if (on) {
+        d = strided_tree(ad);
+        e = strided_tree(ae);
       }
       part[g][ll] = d;
       __syncthreads();
@@ -569,13 +662,34 @@ __global__ void __launch_bounds__(TPB)
     }
   }
   const int cap = L.cap[b.level];
+  auto cross_at = [&](int c) {
+    return packed ? (nact <= 4 ? part[0][c] : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c])
+                  : (nact == 1 ? cross : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c]);
+  };
+  // The first term of each sum below and this thread's column-phase pivot
+  // slot are loaded together, one memory latency for all; the loops take
+  // the terms past TPB (same order: the first term, then the loop's).
+  // Absent terms are a +0.0 the compiler cannot fold (see strided_load).
+  const double zero = 0.0 * (double)b.m;
+  double x0 = zero, y0 = zero, z0 = zero;
+  if (pend) {
+    if (t < nact) x0 = cross_at(t);
+    if (t < b.row_nwg) y0 = slot_ptr(L, b, t)[k - 1];
+    if (t < b.col_nwg) z0 = slot_ptr(L, b, b.row_nwg + t)[k - 1];
+  }
+  PivRed p0;
+  p0.v = -1.0;
+  p0.i = -1;
+  p0.first = -1;
+  if (j >= 0 && t < b.col_nwg) read_piv(slot_ptr(L, b, b.row_nwg + t), cap, p0.v, p0.i, p0.first);
   double crossSum = 0.0, nv2 = 0.0, nw2 = 0.0;
   if (pend) {
-    for (int c = t; c < nact; c += TPB)
-      crossSum += packed ? (nact <= 4 ? part[0][c] : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c])
-                         : (nact == 1 ? cross : fin_part[(size_t)bidx * MAX_FIN_CHUNKS + c]);
-    for (int w = t; w < b.row_nwg; w += TPB) nv2 += slot_ptr(L, b, w)[k - 1];
-    for (int w = t; w < b.col_nwg; w += TPB) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
+    crossSum += x0;
+    for (int c = t + TPB; c < nact; c += TPB) crossSum += cross_at(c);
+    nv2 += y0;
+    for (int w = t + TPB; w < b.row_nwg; w += TPB) nv2 += slot_ptr(L, b, w)[k - 1];
+    nw2 += z0;
+    for (int w = t + TPB; w < b.col_nwg; w += TPB) nw2 += slot_ptr(L, b, b.row_nwg + w)[k - 1];
     crossSum = wg_sum(crossSum, red);
     nv2 = wg_sum(nv2, red);
     nw2 = wg_sum(nw2, red);
@@ -586,7 +700,7 @@ __global__ void __launch_bounds__(TPB)
   cr.v = -1.0;
   cr.i = -1;
   cr.first = -1;
-  if (j >= 0) cr = reduce_slots(L, b, b.row_nwg, b.col_nwg, pr_sh);
+  if (j >= 0) cr = reduce_slots(L, b, b.row_nwg, b.col_nwg, pr_sh, p0);
   if (j == -1) {  // first unchosen row for a jump
     if (t == 0) first_row = 0x7fffffff;
     __syncthreads();

@@ -130,6 +130,9 @@ __global__ void __launch_bounds__(TPB)
     __syncthreads();
     if (r0 + PR < rend) load(r0 + PR);
     const int nr = min(PR, rend - r0);
+    // two rows per trip: the pair (sh, sl) alternates between two register
+    // sets instead of being copied back every row
+#pragma unroll 2
     for (int rr = 0; rr < nr; rr++) {
       double va[CA], xb[JB];
 #pragma unroll
@@ -231,11 +234,30 @@ __global__ void __launch_bounds__(TPB)
     }
     __syncthreads();
     if (row < d.rows) {
+      // the multipliers of four terms in registers while the next four load
+      // (a load per term would wait out a memory latency every term).  The
+      // terms in order; the group's terms past nc have a zero multiplier and
+      // zero staged terms, which leave (sh, sl) exactly unchanged (neither
+      // is ever -0.0: both start at +0.0 and round to nearest), so the
+      // groups need no test per term and the pairs alternate between
+      // register sets instead of being copied back every term.
+      static_assert(AK % 4 == 0, "staged terms in groups of four");
       const gptr<const double> a = to_global(d.A) + row + (size_t)c0 * d.lda;
-      for (int cc = 0; cc < nc; cc++) {
-        const double av = a[(size_t)cc * d.lda];
+      auto load4 = [&](int c, double(&v)[4]) {
 #pragma unroll
-        for (int jj = 0; jj < AJ; jj++) dot2_step_lo(av, Th[cc][jj], Tl[cc][jj], sh[jj], sl[jj]);
+        for (int u = 0; u < 4; u++) v[u] = c + u < nc ? a[(size_t)(c + u) * d.lda] : 0.0;
+      };
+      double cur[4], nxt[4];
+      load4(0, cur);
+      for (int cc = 0; cc < nc; cc += 4) {
+        load4(cc + 4, nxt);
+#pragma unroll
+        for (int u = 0; u < 4; u++)
+#pragma unroll
+          for (int jj = 0; jj < AJ; jj++)
+            dot2_step_lo(cur[u], Th[cc + u][jj], Tl[cc + u][jj], sh[jj], sl[jj]);
+#pragma unroll
+        for (int u = 0; u < 4; u++) cur[u] = nxt[u];
       }
     }
     __syncthreads();
