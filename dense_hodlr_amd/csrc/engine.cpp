@@ -9,6 +9,7 @@
 #include <map>
 #include <mutex>
 #include <cmath>
+#include <cstdint>
 #include <cstdlib>
 #include <cstdio>
 #include <cstring>
@@ -1022,6 +1023,11 @@ void Engine::aca_compress(std::vector<int>& blk_rank) {
       if (nchunk[b] > MAX_FIN_CHUNKS) throw std::runtime_error("ACA rank capacity exceeds 16384");
     }
     for (int b : active) {
+      // the finalize addresses a block's partial slots with 32-bit byte
+      // offsets from its first slot
+      if ((size_t)(blk[b].row_nwg + blk[b].col_nwg) * (cap[blk[b].level] + 4) * sizeof(double) >
+          (size_t)INT32_MAX)
+        throw std::runtime_error("ACA: a block's partial slots exceed 2 GiB (rank capacity too large)");
       blk[b].row_wg0 = (int)m_row.size();
       for (int w = 0; w < blk[b].row_nwg; w++) m_row.push_back(b);
       blk[b].col_wg0 = (int)m_col.size();
