@@ -66,14 +66,14 @@ struct ProjShape {
   static constexpr int TJ = TPB / TC;
   static constexpr int CT = TC * CA;  // c tile
   static constexpr int JT = TJ * JB;  // j tile
-  static constexpr int PR = CT + JT > 128 ? 16 : 32;
+  static constexpr int PR = CT + JT >= 128 ? 16 : 32;
 };
 using ProjWide = ProjShape<16, 4, 4>;   // 64 x 64, many right-hand sides
 using ProjMid = ProjShape<16, 4, 1>;    // 64 x 16
 using ProjNarrow = ProjShape<64, 4, 1>; // 256 x 4, up to 4 right-hand sides
 
 template <int TC, int CA, int JB>
-__global__ void __launch_bounds__(TPB)
+__global__ void __launch_bounds__(TPB) __attribute__((amdgpu_waves_per_eu(3)))
     dd_proj_kernel(const DdProjDesc* __restrict__ descs, int count, double* __restrict__ ph,
                    double* __restrict__ pl) {
   using S = ProjShape<TC, CA, JB>;
