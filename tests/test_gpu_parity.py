@@ -89,6 +89,31 @@ def test_factors_match_oracle_many_blocks(oracle):
     assert not bad, bad[:10]
 
 
+def test_factors_match_oracle_multi_workgroup_finalize(oracle):
+    """Bit-for-bit factors and ranks when the top-level blocks finalize over
+    several workgroups: 2D Gaussian N=16384 in kd order, l=0.05, leaf 256.
+    Once only the two 8192-row top blocks are left, their panel rows spread
+    to one per thread (32 row and 32 column workgroups each), so a block has
+    64 partial slots (more than the 32 of a one-workgroup finalize) and,
+    past rank 65, several finalize workgroups whose partials the last one
+    combines."""
+    n = 16384
+    pts = I.points_2d_kd(n, 515)
+    _case("factors gauss2d_16384 leaf 256")
+    g, o = _pair(oracle, n, pts, "gaussian", 0.05, 1e-4, 256, 1e-10)
+    g.storeLRinTree()
+    o.compress()
+    assert np.array_equal(g.nodes(), o.nodes())
+    assert g.nodes()[0, 4] > 65 and g.nodes()[0, 5] > 65  # several finalize chunks
+    bad = []
+    for idx in range(len(o.nodes())):
+        for which in range(4):
+            a, b = g.factor(idx, which), o.factor(idx, which)
+            if a.shape != b.shape or not np.array_equal(a, b):
+                bad.append((idx, which))
+    assert not bad, bad[:10]
+
+
 def test_recsm_matches_oracle(gauss1d, oracle):
     cfg, g, o = gauss1d
     _case("recSM gauss1d_4096")
