@@ -208,9 +208,11 @@ __global__ void __launch_bounds__(TPB)
   const DdApplyDesc d = descs[find_desc(descs, count, (int)blockIdx.x, &DdApplyDesc::wg0)];
   const gptr<const double> tH = to_global(d.t_hi), tL = to_global(d.t_lo);
   const gptr<double> yH = to_global(d.y_hi), yL = to_global(d.y_lo);
-  const int nrt = (d.rows + TPB - 1) / TPB;
   const int w = (int)blockIdx.x - d.wg0;
-  const int row0 = (w % nrt) * TPB, j0 = (w / nrt) * AJ;
+  // the right-hand-side groups of one row tile are consecutive workgroups,
+  // which run together and share the tile's rows of A through the L2
+  const int njt = (d.nrhs + AJ - 1) / AJ;
+  const int row0 = (w / njt) * TPB, j0 = (w % njt) * AJ;
   const int t = threadIdx.x, row = row0 + t;
   const int nj = min(AJ, d.nrhs - j0);
   double sh[AJ], sl[AJ];
